@@ -1,0 +1,712 @@
+/*
+ * FP64 CPU oracle of the split-step GPE engine (TEST INFRASTRUCTURE ONLY:
+ * loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+ * never by the product path).
+ *
+ * Restatement of orginal/gpe_engine.cu (step, energy, normalisation, quantum
+ * friction, density, currents) and of dev/gpe_engine.cu (chemical potential,
+ * vortex imprint, evolve_vortex), with the physics hooks of
+ * orginal/gpe_user_defined.h and dev/gpe_user_defined.h.  Every array is FP64
+ * and every sum is taken in a fixed order that does not depend on the thread
+ * count, so results are reproducible run to run.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#include <omp.h>
+
+#include "gpe_oracle.h"
+
+typedef struct { double re, im; } cd;
+
+static inline cd cmul(cd a, cd b) { cd r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; return r; }
+static inline cd cscale(cd a, double s) { cd r = {s * a.re, s * a.im}; return r; }
+/* exp(z) = e^Re (cos Im, sin Im): dev/gpe_complex.cuh:167-175, orginal/gpe_engine.cu:358-366 */
+static inline cd cexp_(cd z) { double t = exp(z.re); cd r = {t * cos(z.im), t * sin(z.im)}; return r; }
+
+#define QF_EPSILON 1.0e-12 /* orginal/gpe_engine.cu:129 */
+#define MAXUP 32           /* orginal/gpe_user_defined.h:45 */
+
+struct gpeo {
+    int nx, ny, nz;
+    size_t n;
+    int interactions, vext;
+    double gamma;
+    double alpha, beta, dt, npart, t0;
+    unsigned it;
+    double qf;            /* stored as qf/(gamma N): orginal/gpe_engine.cu:642 */
+    double up[MAXUP];     /* d_user_param */
+    cd c;                 /* d_step_coeff = gamma dt/2 / (-beta + i alpha) */
+    double *kx, *ky, *kz;
+    cd *Kx, *Ky, *Kz;     /* Kz includes 1/N (d_exp_kkz2_over_nxyz) */
+    cd *psi, *psi2;
+    double *wrkR;         /* V0 of the predictor (d_wrk2R) */
+    double *qfpot;        /* d_wrk3R */
+    cd *wrk;              /* FFT scratch (d_wrk3C / d_wrk2 in the reference) */
+    double *varr;         /* VEXT 3 array */
+    double vx0, vy0;
+    int vQ;
+};
+
+/* ------------------------------------------------------------------ hooks */
+
+/* gpe_external_potential: VEXT 0 harmonic (orginal/gpe_user_defined.h:79-101,
+ * dev/gpe_user_defined.h:488-513), VEXT 1 Woods-Saxon tube
+ * (dev/gpe_user_defined.h:533-554), VEXT 2 cosine lattice and VEXT 3 array
+ * (additions of this engine). */
+static inline double vext_val(const gpeo_t *e, unsigned ix, unsigned iy, unsigned iz, size_t ixyz)
+{
+    const double *p = e->up;
+    switch (e->vext) {
+    case 0: {
+        double x = (double)ix - 1.0 * (e->nx / 2);
+        double y = (double)iy - 1.0 * (e->ny / 2);
+        double z = (double)iz - 1.0 * (e->nz / 2);
+        return 0.5 * x * x * p[0] * p[0] + 0.5 * y * y * p[1] * p[1] + 0.5 * z * z * p[2] * p[2];
+    }
+    case 1: {
+        double x = (double)ix - 1.0 * (e->nx / 2);
+        double y = (double)iy - 1.0 * (e->ny / 2);
+        double r = sqrt(x * x + y * y);
+        return p[0] * (1. - 1. / (1 + exp((r - p[2]) / p[1])));
+    }
+    case 2: {
+        const double tp = 2.0 * M_PI * p[1];
+        return p[0] * (cos(tp * (double)ix / (double)e->nx) + cos(tp * (double)iy / (double)e->ny) +
+                       cos(tp * (double)iz / (double)e->nz));
+    }
+    case 3:
+        return e->varr ? e->varr[ixyz] : 0.0;
+    default:
+        return 0.0;
+    }
+}
+
+#define XI_ 0.37
+#define CONTACT_ 0.901
+
+/* gpe_dEDFdn: INTERACTIONS 0 unitary Fermi gas (orginal/gpe_user_defined.h:122-126,
+ * dev/gpe_user_defined.h:131-135), 1 ETF BEC side (dev 214-243), -1 contact
+ * bosons (dev 317-322), -2 dipolar local part (dev 400-404). */
+static inline double dedfdn(const gpeo_t *e, double rho)
+{
+    switch (e->interactions) {
+    case 0:
+        return 0.37 * pow(3.0 * M_PI * M_PI * rho, 2.0 / 3.0) / 2.0;
+    case 1: {
+        const double a = e->up[3];
+        if (rho > 1e-15) {
+            const double kF = pow(3.0 * M_PI * M_PI * rho, 1.0 / 3.0);
+            const double eF = 0.5 * kF * kF;
+            const double x = (kF > DBL_MIN) ? 1.0 / (a * kF) : DBL_MAX;
+            const double D = (XI_ + x * (1.0 + CONTACT_) + 3.0 * M_PI * XI_ * x * x);
+            return XI_ * eF * (XI_ + 0.8 * x) / D +
+                   0.2 * XI_ * eF * (XI_ + x) * x * ((1.0 + CONTACT_) + 6.0 * M_PI * XI_ * x) / (D * D) -
+                   1.0 / (2.0 * a * a);
+        }
+        return -1.0 / (2.0 * a * a);
+    }
+    case -1:
+        return 4 * M_PI * e->up[3] * rho;
+    case -2:
+        return 0.;
+    default:
+        return 0.;
+    }
+}
+
+/* gpe_EDF: orginal/gpe_user_defined.h:109-114, dev/gpe_user_defined.h:118-123,
+ * 196-206, 305-309, 387-392 */
+static inline double edf(const gpeo_t *e, double rho)
+{
+    switch (e->interactions) {
+    case 0:
+        return 0.37 * 0.6 * rho * pow(3.0 * M_PI * M_PI * rho, 2.0 / 3.0) / 2.;
+    case 1: {
+        const double a = e->up[3];
+        const double kF = pow(3.0 * M_PI * M_PI * rho, 1.0 / 3.0);
+        const double eF = 0.5 * kF * kF;
+        const double x = (kF > DBL_MIN) ? 1.0 / (a * kF) : DBL_MAX;
+        return 0.6 * eF * rho * XI_ * (XI_ + x) / (XI_ + x * (1.0 + CONTACT_) + 3.0 * M_PI * XI_ * x * x) -
+               rho / (2.0 * a * a);
+    }
+    case -1:
+        return 2 * M_PI * e->up[3] * rho * rho;
+    case -2:
+        return 2 * M_PI * e->up[3] * rho * rho;
+    default:
+        return 0.;
+    }
+}
+
+/* gpe_density: orginal/gpe_engine.cu:384-387 */
+static inline double density(const gpeo_t *e, cd p) { return e->gamma * (p.re * p.re + p.im * p.im); }
+
+/* gpe_modify_psi: identity (orginal/gpe_user_defined.h:66-69) */
+static inline cd modify_psi(cd p) { return p; }
+
+/* ---------------------------------------------------------- utilities */
+
+#define IDX3(ix, iy, iz) (((size_t)(ix) * e->ny + (iy)) * e->nz + (iz))
+
+/* sum of f over the lattice in a fixed order: per-x-plane partial sums,
+ * then the planes in order (independent of the thread count) */
+static double *plane_buf(const gpeo_t *e) { return (double *)malloc(sizeof(double) * (size_t)e->nx); }
+
+static double ordered_sum_planes(const double *planes, int nx)
+{
+    double s = 0.0;
+    for (int i = 0; i < nx; i++) s += planes[i];
+    return s;
+}
+
+static double sum_density(const gpeo_t *e, const cd *p)
+{
+    double *pl = plane_buf(e);
+    const size_t pn = (size_t)e->ny * e->nz;
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++) {
+        double s = 0.0;
+        const cd *q = p + (size_t)ix * pn;
+        for (size_t j = 0; j < pn; j++) s += density(e, q[j]);
+        pl[ix] = s;
+    }
+    double r = ordered_sum_planes(pl, e->nx);
+    free(pl);
+    return r;
+}
+
+/* gpe_normalize: orginal/gpe_engine.cu:684-692 (density, tree sum, scale by
+ * sqrt(npart/sum)); returns the sum before scaling */
+static double normalize(gpeo_t *e, cd *p)
+{
+    double sum = sum_density(e, p);
+    double f = sqrt(e->npart / sum);
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < e->n; i++) p[i] = cscale(p[i], f);
+    return sum;
+}
+
+static void fft3(gpeo_t *e, cd *a, int sign) { gpeo_fft3d(e->nx, e->ny, e->nz, (double *)a, sign); }
+
+/* reciprocal lattice and kinetic factors: orginal/gpe_engine.cu:446-517 and
+ * gpe_change_alpha_beta 572-610; dev/gpe_engine.cu:126-281 */
+static void fill_k(double *k, int n)
+{
+    int i, j;
+    for (i = 0; i <= n / 2 - 1; i++) k[i] = 2. * M_PI / n * (double)i;
+    j = -i;
+    for (i = n / 2; i < n; i++) { k[i] = 2. * M_PI / n * (double)j; j++; }
+}
+
+/* complex exp of a complex argument, as std::exp(std::complex<double>) */
+static cd cexp_host(double re, double im) { double t = exp(re); cd r = {t * cos(im), t * sin(im)}; return r; }
+
+static void build_kinetic(gpeo_t *e)
+{
+    /* c3 = (gamma*0.5*dt) / (-beta + i alpha)  (std::complex division) */
+    double a = e->gamma * 0.5 * e->dt;
+    double br = -1.0 * e->beta, bi = e->alpha;
+    double den = br * br + bi * bi;
+    e->c.re = (a * br) / den;
+    e->c.im = (-a * bi) / den;
+    const double g2 = e->gamma * e->gamma;
+    for (int i = 0; i < e->nx; i++) {
+        double k2 = e->kx[i] * e->kx[i];
+        e->Kx[i] = cexp_host(e->c.re * k2 / g2, e->c.im * k2 / g2);
+    }
+    for (int i = 0; i < e->ny; i++) {
+        double k2 = e->ky[i] * e->ky[i];
+        e->Ky[i] = cexp_host(e->c.re * k2 / g2, e->c.im * k2 / g2);
+    }
+    for (int i = 0; i < e->nz; i++) {
+        double k2 = e->kz[i] * e->kz[i];
+        cd t = cexp_host(e->c.re * k2 / g2, e->c.im * k2 / g2);
+        e->Kz[i] = cscale(t, 1.0 / (double)e->n);
+    }
+}
+
+/* ------------------------------------------------------------- lifecycle */
+
+gpeo_t *gpeo_create(int nx, int ny, int nz, int interactions, int vext, int gpe_for, double alpha,
+                    double beta, double dt, double npart)
+{
+    if (nx < 2 || ny < 2 || nz < 2) return NULL;
+    gpeo_t *e = (gpeo_t *)calloc(1, sizeof(gpeo_t));
+    e->nx = nx; e->ny = ny; e->nz = nz;
+    e->n = (size_t)nx * ny * nz;
+    e->interactions = interactions;
+    e->vext = vext;
+    if (gpe_for == 1) e->gamma = 1.0;
+    else if (gpe_for == 2) e->gamma = 2.0;
+    else e->gamma = (interactions == 0 || interactions == 1) ? 2.0 : 1.0;
+    e->alpha = alpha; e->beta = beta; e->dt = dt; e->npart = npart;
+    e->t0 = 0.0; e->it = 0; e->qf = 0.0;
+    e->kx = (double *)malloc(sizeof(double) * nx);
+    e->ky = (double *)malloc(sizeof(double) * ny);
+    e->kz = (double *)malloc(sizeof(double) * nz);
+    fill_k(e->kx, nx); fill_k(e->ky, ny); fill_k(e->kz, nz);
+    e->Kx = (cd *)malloc(sizeof(cd) * nx);
+    e->Ky = (cd *)malloc(sizeof(cd) * ny);
+    e->Kz = (cd *)malloc(sizeof(cd) * nz);
+    build_kinetic(e);
+    e->psi = (cd *)calloc(e->n, sizeof(cd));
+    e->psi2 = (cd *)calloc(e->n, sizeof(cd));
+    e->wrkR = (double *)calloc(e->n, sizeof(double));
+    e->wrk = (cd *)calloc(e->n, sizeof(cd));
+    e->qfpot = NULL;
+    e->varr = NULL;
+    e->vQ = 1;
+    return e;
+}
+
+void gpeo_destroy(gpeo_t *e)
+{
+    if (!e) return;
+    free(e->kx); free(e->ky); free(e->kz);
+    free(e->Kx); free(e->Ky); free(e->Kz);
+    free(e->psi); free(e->psi2); free(e->wrkR); free(e->wrk);
+    free(e->qfpot); free(e->varr);
+    free(e);
+}
+
+/* orginal/gpe_engine.cu:560-613 */
+int gpeo_change_alpha_beta(gpeo_t *e, double alpha, double beta)
+{
+    e->alpha = alpha; e->beta = beta;
+    build_kinetic(e);
+    return 0;
+}
+
+/* orginal/gpe_engine.cu:615-624 */
+int gpeo_set_time(gpeo_t *e, double t0) { e->t0 = t0; e->it = 0; return 0; }
+
+/* orginal/gpe_engine.cu:626-635; copies `size` values and zeroes the rest
+ * (the reference copies MAX_USER_PARAMS regardless of size) */
+int gpeo_set_user_params(gpeo_t *e, int size, const double *params)
+{
+    if (size > MAXUP) return -9;
+    if (size < 0) return -1;
+    memset(e->up, 0, sizeof(e->up));
+    for (int i = 0; i < size; i++) e->up[i] = params[i];
+    return 0;
+}
+
+/* orginal/gpe_engine.cu:637-662 */
+int gpeo_set_quantum_friction_coeff(gpeo_t *e, double qf)
+{
+    if (qf != 0.0) {
+        e->qf = qf / (e->gamma * (double)e->n);
+        if (!e->qfpot) e->qfpot = (double *)calloc(e->n, sizeof(double));
+    } else {
+        e->qf = 0.0;
+        free(e->qfpot);
+        e->qfpot = NULL;
+    }
+    return 0;
+}
+
+/* dev/gpe_engine.cu:465-476 */
+int gpeo_set_vortex(gpeo_t *e, double x0, double y0, int Q) { e->vx0 = x0; e->vy0 = y0; e->vQ = Q; return 0; }
+
+int gpeo_set_vext_array(gpeo_t *e, const double *v)
+{
+    if (!e->varr) e->varr = (double *)malloc(sizeof(double) * e->n);
+    memcpy(e->varr, v, sizeof(double) * e->n);
+    return 0;
+}
+
+/* orginal/gpe_engine.cu:699-717 */
+int gpeo_set_psi(gpeo_t *e, double t, const double *psi)
+{
+    e->t0 = t; e->it = 0;
+    memcpy(e->psi, psi, sizeof(cd) * e->n);
+    return 0;
+}
+
+int gpeo_get_psi(gpeo_t *e, double *t, double *psi)
+{
+    memcpy(psi, e->psi, sizeof(cd) * e->n);
+    *t = e->t0 + e->dt * e->it;
+    return 0;
+}
+
+int gpeo_normalize_psi(gpeo_t *e) { normalize(e, e->psi); return 0; }
+
+/* dev/gpe_engine.cu:679-689 */
+int gpeo_get_norm(gpeo_t *e, double *norm) { *norm = sum_density(e, e->psi); return 0; }
+
+/* ----------------------------------------------------------- step parts */
+
+/* __gpe_multiply_by_expT__: orginal/gpe_engine.cu:880-895 */
+static void multiply_by_expT(gpeo_t *e, cd *a)
+{
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++)
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                cd w = a[i];
+                w = cmul(w, e->Kx[ix]);
+                w = cmul(w, e->Ky[iy]);
+                w = cmul(w, e->Kz[iz]);
+                a[i] = w;
+            }
+}
+
+/* gpe_compute_qf_potential: orginal/gpe_engine.cu:897-937 */
+static void compute_qf_potential(gpeo_t *e, const cd *psi, cd *wrk, double *qfpot)
+{
+    memcpy(wrk, psi, sizeof(cd) * e->n);
+    fft3(e, wrk, -1);
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++)
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                double f = e->qf * (e->kx[ix] * e->kx[ix] + e->ky[iy] * e->ky[iy] + e->kz[iz] * e->kz[iz]);
+                wrk[i] = cscale(wrk[i], f);
+            }
+    fft3(e, wrk, +1);
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < e->n; i++) {
+        cd l = psi[i];
+        double r = density(e, l);
+        /* cplxMulI(cplxConj(l), wrk) = l.re*w.im + (-l.im)*w.re */
+        qfpot[i] = (l.re * wrk[i].im + (-l.im) * wrk[i].re) / (r + QF_EPSILON);
+    }
+}
+
+/* __gpe_exp_Vstep1_ / _qf_: orginal/gpe_engine.cu:723-772 */
+static void vstep1(gpeo_t *e, unsigned it, const cd *psi_in, cd *psi_out, double *wrkR, const double *qf)
+{
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++)
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                cd l = modify_psi(psi_in[i]);
+                double r = density(e, l);
+                double lv = vext_val(e, ix, iy, iz, i) + dedfdn(e, r);
+                if (qf) lv = lv + qf[i];
+                wrkR[i] = lv;
+                cd ex = cexp_(cscale(e->c, lv));
+                psi_out[i] = cmul(l, ex);
+            }
+}
+
+/* __gpe_exp_Vstep2_: orginal/gpe_engine.cu:774-801 (wrkC aliases psi_in) */
+static void vstep2(gpeo_t *e, unsigned it, cd *psi_in, cd *psi_out, const double *wrkR, cd *wrkC)
+{
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++)
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                cd l = psi_in[i];
+                double lv = wrkR[i];
+                cd ex = cexp_(cscale(e->c, lv));
+                l = cmul(l, ex);
+                double r = density(e, l);
+                lv = 0.5 * (lv + vext_val(e, ix, iy, iz, i) + dedfdn(e, r));
+                ex = cexp_(cscale(e->c, lv));
+                wrkC[i] = ex;
+                l = modify_psi(psi_out[i]);
+                psi_out[i] = cmul(l, ex);
+            }
+}
+
+/* __gpe_exp_Vstep2_part1_: orginal/gpe_engine.cu:803-815 */
+static void vstep2_part1(gpeo_t *e, cd *psi, const double *wrkR)
+{
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < e->n; i++) psi[i] = cmul(psi[i], cexp_(cscale(e->c, wrkR[i])));
+}
+
+/* __gpe_exp_Vstep2_part2_ / _qf_: orginal/gpe_engine.cu:817-868 */
+static void vstep2_part2(gpeo_t *e, unsigned it, cd *psi_in, cd *psi_out, const double *wrkR, cd *wrkC,
+                         const double *qf)
+{
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++)
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                cd l = psi_in[i];
+                double lv = wrkR[i];
+                double r = density(e, l);
+                if (qf) lv = 0.5 * (lv + vext_val(e, ix, iy, iz, i) + dedfdn(e, r) + qf[i]);
+                else lv = 0.5 * (lv + vext_val(e, ix, iy, iz, i) + dedfdn(e, r));
+                cd ex = cexp_(cscale(e->c, lv));
+                wrkC[i] = ex;
+                l = modify_psi(psi_out[i]);
+                psi_out[i] = cmul(l, ex);
+            }
+}
+
+/* __gpe_exp_Vstep3_: orginal/gpe_engine.cu:870-878 */
+static void vstep3(gpeo_t *e, cd *psi, const cd *W)
+{
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < e->n; i++) psi[i] = cmul(psi[i], W[i]);
+}
+
+/* __gpe_imprint_vortexline_zdir_: dev/gpe_engine.cu:485-511.  Phase
+ * Q*atan2(x, y) (arguments in this order), modulus kept. */
+static void imprint_vortex(gpeo_t *e, cd *psi)
+{
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++)
+        for (int iy = 0; iy < e->ny; iy++) {
+            double x = (double)ix - 1.0 * (e->nx / 2) - e->vx0;
+            double y = (double)iy - 1.0 * (e->ny / 2) - e->vy0;
+            double ph = atan2(x, y) * (double)e->vQ;
+            double cp = cos(ph), sp = sin(ph);
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                double a = hypot(psi[i].re, psi[i].im);
+                psi[i].re = a * cp;
+                psi[i].im = a * sp;
+            }
+        }
+}
+
+/* One split step shared by gpe_evolve_qf (orginal/gpe_engine.cu:948-1013,
+ * dev/gpe_engine.cu:1035-1106) and gpe_evolve_vortex (dev/gpe_engine.cu:1120-1199). */
+static void one_step(gpeo_t *e, int vortex, double *mu)
+{
+    if (vortex) imprint_vortex(e, e->psi);
+
+    if (e->qf != 0.0) {
+        compute_qf_potential(e, e->psi, e->wrk, e->qfpot);
+        vstep1(e, e->it, e->psi, e->psi2, e->wrkR, e->qfpot);
+    } else {
+        vstep1(e, e->it, e->psi, e->psi2, e->wrkR, NULL);
+    }
+
+    fft3(e, e->psi2, -1);
+    multiply_by_expT(e, e->psi2);
+    fft3(e, e->psi2, +1);
+
+    if (e->beta == 0.0 && e->qf == 0.0) {
+        vstep2(e, e->it, e->psi2, e->psi, e->wrkR, e->psi2);
+    } else {
+        vstep2_part1(e, e->psi2, e->wrkR);
+        if (e->beta != 0.0) {
+            normalize(e, e->psi2);
+            if (vortex) imprint_vortex(e, e->psi);
+        }
+        if (e->qf != 0.0) {
+            compute_qf_potential(e, e->psi, e->wrk, e->qfpot);
+            vstep2_part2(e, e->it, e->psi2, e->psi, e->wrkR, e->psi2, e->qfpot);
+        } else {
+            vstep2_part2(e, e->it, e->psi2, e->psi, e->wrkR, e->psi2, NULL);
+        }
+    }
+
+    fft3(e, e->psi, -1);
+    multiply_by_expT(e, e->psi);
+    fft3(e, e->psi, +1);
+
+    vstep3(e, e->psi, e->psi2);
+
+    if (e->beta != 0.0) {
+        double norm = normalize(e, e->psi);
+        if (mu) *mu = -.5 * log(norm / e->npart) / e->dt;  /* dev/gpe_engine.cu:1097-1102 */
+    }
+    e->it = e->it + 1;
+}
+
+int gpeo_evolve_qf(gpeo_t *e, int nt, double *mu)
+{
+    for (int i = 0; i < nt; i++) one_step(e, 0, mu);
+    return 0;
+}
+
+int gpeo_evolve_vortex(gpeo_t *e, int nt, double *mu)
+{
+    for (int i = 0; i < nt; i++) one_step(e, 1, mu);
+    return 0;
+}
+
+/* gpe_energy: orginal/gpe_engine.cu:1076-1113 */
+int gpeo_energy(gpeo_t *e, double *t, double *ekin, double *eint, double *eext)
+{
+    *t = e->t0 + e->dt * e->it;
+    if (e->beta == 0.0) normalize(e, e->psi);
+
+    double *pl_ext = plane_buf(e), *pl_int = plane_buf(e), *pl_kin = plane_buf(e);
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++) {
+        double se = 0.0, si = 0.0;
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                double r = density(e, e->psi[i]);
+                se += r * vext_val(e, ix, iy, iz, i);
+                si += edf(e, r);
+            }
+        pl_ext[ix] = se;
+        pl_int[ix] = si;
+    }
+    *eext = ordered_sum_planes(pl_ext, e->nx);
+    *eint = ordered_sum_planes(pl_int, e->nx);
+
+    memcpy(e->psi2, e->psi, sizeof(cd) * e->n);
+    fft3(e, e->psi2, -1);
+    const double norm = 2.0 * e->gamma * (double)e->n;
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++)
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                double f = (e->kx[ix] * e->kx[ix] + e->ky[iy] * e->ky[iy] + e->kz[iz] * e->kz[iz]) / norm;
+                e->psi2[i] = cscale(e->psi2[i], f);
+            }
+    fft3(e, e->psi2, +1);
+    const size_t pn = (size_t)e->ny * e->nz;
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++) {
+        double s = 0.0;
+        for (size_t j = 0; j < pn; j++) {
+            size_t i = (size_t)ix * pn + j;
+            s += e->psi[i].re * e->psi2[i].re + e->psi[i].im * e->psi2[i].im;
+        }
+        pl_kin[ix] = s;
+    }
+    *ekin = ordered_sum_planes(pl_kin, e->nx);
+    free(pl_ext); free(pl_int); free(pl_kin);
+    return 0;
+}
+
+/* gpe_get_density: orginal/gpe_engine.cu:1115-1123 */
+int gpeo_get_density(gpeo_t *e, double *t, double *rho)
+{
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < e->n; i++) rho[i] = density(e, e->psi[i]);
+    *t = e->t0 + e->dt * e->it;
+    return 0;
+}
+
+/* gpe_get_currents: orginal/gpe_engine.cu:1125-1212 */
+int gpeo_get_currents(gpeo_t *e, double *t, double *jx, double *jy, double *jz)
+{
+    *t = e->t0 + e->dt * e->it;
+    cd *F = e->psi2;
+    memcpy(F, e->psi, sizeof(cd) * e->n);
+    fft3(e, F, -1);
+    double *outs[3] = {jx, jy, jz};
+    for (int a = 0; a < 3; a++) {
+#pragma omp parallel for schedule(static)
+        for (int ix = 0; ix < e->nx; ix++)
+            for (int iy = 0; iy < e->ny; iy++)
+                for (int iz = 0; iz < e->nz; iz++) {
+                    size_t i = IDX3(ix, iy, iz);
+                    double k = (a == 0) ? e->kx[ix] : (a == 1) ? e->ky[iy] : e->kz[iz];
+                    e->wrk[i] = cscale(F[i], k / (e->gamma * (double)e->n));
+                }
+        fft3(e, e->wrk, +1);
+        double *o = outs[a];
+#pragma omp parallel for schedule(static)
+        for (size_t i = 0; i < e->n; i++) o[i] = e->psi[i].re * e->wrk[i].re + e->psi[i].im * e->wrk[i].im;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------- introspection */
+
+double gpeo_gamma(const gpeo_t *e) { return e->gamma; }
+
+int gpeo_tables(const gpeo_t *e, double *kx, double *ky, double *kz, double *Kx, double *Ky, double *Kz,
+                double *step_coeff)
+{
+    if (kx) memcpy(kx, e->kx, sizeof(double) * e->nx);
+    if (ky) memcpy(ky, e->ky, sizeof(double) * e->ny);
+    if (kz) memcpy(kz, e->kz, sizeof(double) * e->nz);
+    if (Kx) memcpy(Kx, e->Kx, sizeof(cd) * e->nx);
+    if (Ky) memcpy(Ky, e->Ky, sizeof(cd) * e->ny);
+    if (Kz) memcpy(Kz, e->Kz, sizeof(cd) * e->nz);
+    if (step_coeff) { step_coeff[0] = e->c.re; step_coeff[1] = e->c.im; }
+    return 0;
+}
+
+int gpeo_vext_value(const gpeo_t *e, int ix, int iy, int iz, double *v)
+{
+    *v = vext_val(e, (unsigned)ix, (unsigned)iy, (unsigned)iz, IDX3(ix, iy, iz));
+    return 0;
+}
+
+int gpeo_dedfdn_value(const gpeo_t *e, double rho, double *dv, double *edfv)
+{
+    if (dv) *dv = dedfdn(e, rho);
+    if (edfv) *edfv = edf(e, rho);
+    return 0;
+}
+
+/* ------------------------------------------------- plaquette windings */
+
+/* quadrant of a non-zero complex number: 0 for arg in [0, pi/2), 1 for
+ * [pi/2, pi), 2 for [pi, 3pi/2), 3 for [3pi/2, 2pi) */
+static inline int quadrant(double x, double y)
+{
+    if (y >= 0.0 && x > 0.0) return 0;
+    if (x <= 0.0 && y > 0.0) return 1;
+    if (y <= 0.0 && x < 0.0) return 2;
+    return 3;
+}
+
+/* signed quarter turns of the principal phase change from a to b */
+static inline int quarter_turns(double ax, double ay, double bx, double by)
+{
+    int d = (quadrant(bx, by) - quadrant(ax, ay)) & 3;
+    if (d == 1) return 1;
+    if (d == 3) return -1;
+    if (d == 0) return 0;
+    double cr = ax * by - ay * bx; /* sign of sin of the phase change */
+    return (cr < 0.0) ? -2 : 2;
+}
+
+long gpeo_plaquette_windings(int nx, int ny, int nz, const double *psi64, const float *psi32, double rho_min,
+                             int8_t *w)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    long count = 0;
+#pragma omp parallel for schedule(static) reduction(+ : count)
+    for (int ix = 0; ix < nx; ix++) {
+        for (int iy = 0; iy < ny; iy++)
+            for (int iz = 0; iz < nz; iz++) {
+                const int jx = (ix + 1) % nx, jy = (iy + 1) % ny, jz = (iz + 1) % nz;
+                size_t cidx[3][4] = {
+                    {((size_t)ix * ny + iy) * nz + iz, ((size_t)jx * ny + iy) * nz + iz,
+                     ((size_t)jx * ny + jy) * nz + iz, ((size_t)ix * ny + jy) * nz + iz},
+                    {((size_t)ix * ny + iy) * nz + iz, ((size_t)ix * ny + jy) * nz + iz,
+                     ((size_t)ix * ny + jy) * nz + jz, ((size_t)ix * ny + iy) * nz + jz},
+                    {((size_t)ix * ny + iy) * nz + iz, ((size_t)ix * ny + iy) * nz + jz,
+                     ((size_t)jx * ny + iy) * nz + jz, ((size_t)jx * ny + iy) * nz + iz}};
+                const size_t here = ((size_t)ix * ny + iy) * nz + iz;
+                for (int o = 0; o < 3; o++) {
+                    double re[4], im[4];
+                    int masked = 0;
+                    for (int c = 0; c < 4; c++) {
+                        size_t q = cidx[o][c];
+                        re[c] = psi64 ? psi64[2 * q] : (double)psi32[2 * q];
+                        im[c] = psi64 ? psi64[2 * q + 1] : (double)psi32[2 * q + 1];
+                        double r2 = re[c] * re[c];
+                        r2 = r2 + im[c] * im[c];
+                        if (!(r2 >= rho_min)) masked = 1;
+                    }
+                    int s = 0;
+                    if (!masked)
+                        for (int c = 0; c < 4; c++) {
+                            int d = (c + 1) & 3;
+                            s += quarter_turns(re[c], im[c], re[d], im[d]);
+                        }
+                    int wind = s / 4;
+                    w[(size_t)o * n + here] = (int8_t)wind;
+                    if (wind != 0) count++;
+                }
+            }
+    }
+    return count;
+}

@@ -1,0 +1,245 @@
+"""Pins the oracle's split step, energy, normalisation, quantum friction,
+density, currents and vortex imprint (orginal/gpe_engine.cu, dev/gpe_engine.cu)
+with checks that the mathematics fixes independently of how the oracle is
+written: closed forms, invariants and a spectral eigen-solve."""
+import numpy as np
+import pytest
+
+# INTERACTIONS / VEXT codes (dev/gpe_user_defined.h:90,169,276,356; 476,520)
+UFG, ETF, CONTACT, DIPOLAR = 0, 1, -1, -2
+HARMONIC, WOODS_SAXON, COSINE, VARRAY = 0, 1, 2, 3
+
+
+def plane_wave(shape, m, amp):
+    nx, ny, nz = shape
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    k = [2 * np.pi * m[0] / nx, 2 * np.pi * m[1] / ny, 2 * np.pi * m[2] / nz]
+    return amp * np.exp(1j * (k[0] * ix + k[1] * iy + k[2] * iz)), k
+
+
+def test_k_grid_and_kinetic_tables(oracle):
+    # orginal/gpe_engine.cu:446-517: k = 2 pi fftfreq(n) with the Nyquist bin negative,
+    # c = gamma dt / 2 / (-beta + i alpha), K_a = exp(c k_a^2 / gamma^2), K_z / N
+    shape = (8, 6, 4)
+    for (alpha, beta, gam) in [(1.0, 0.0, 2), (0.0, 1.0, 1), (0.7, 0.3, 2)]:
+        e = oracle.OracleEngine(shape, alpha, beta, 0.025, 1000.0, UFG, HARMONIC, gam)
+        t = e.tables()
+        for n, kk in zip(shape, (t["kx"], t["ky"], t["kz"])):
+            assert np.allclose(kk, 2 * np.pi * np.fft.fftfreq(n), rtol=0, atol=1e-15)
+            assert kk[n // 2] < 0
+        c = gam * 0.025 / 2 / complex(-beta, alpha)
+        assert abs(t["c"] - c) < 1e-16
+        N = np.prod(shape)
+        assert np.allclose(t["Kx"], np.exp(c * t["kx"] ** 2 / gam**2), rtol=1e-15, atol=0)
+        assert np.allclose(t["Kz"], np.exp(c * t["kz"] ** 2 / gam**2) / N, rtol=1e-15, atol=0)
+
+
+def test_hooks_closed_forms(oracle):
+    shape = (16, 12, 10)
+    e = oracle.OracleEngine(shape, 1.0, 0.0, 0.01, 100.0, UFG, HARMONIC)
+    e.set_user_params([0.3, 0.2, 0.1])
+    # trap centred at n/2 (orginal/gpe_user_defined.h:91-93)
+    assert e.vext(8, 6, 5) == 0.0
+    assert abs(e.vext(0, 0, 0) - 0.5 * (64 * 0.09 + 36 * 0.04 + 25 * 0.01)) < 1e-14
+    # UFG: dEDF/dn = xi/2 (3 pi^2 n)^(2/3), EDF = 3/5 n dEDF/dn; derivative consistency
+    for rho in (1e-3, 0.1, 2.5):
+        dv, ed = e.dedfdn(rho)
+        assert abs(dv - 0.37 / 2 * (3 * np.pi**2 * rho) ** (2 / 3)) < 1e-14 * dv
+        h = rho * 1e-5
+        num = (e.dedfdn(rho + h)[1] - e.dedfdn(rho - h)[1]) / (2 * h)
+        assert abs(num - dv) < 1e-8 * dv
+    c = oracle.OracleEngine(shape, 1.0, 0.0, 0.01, 100.0, CONTACT, HARMONIC)
+    c.set_user_params([0.0, 0.0, 0.0, 0.02])
+    dv, ed = c.dedfdn(0.5)
+    assert abs(dv - 4 * np.pi * 0.02 * 0.5) < 1e-15 and abs(ed - 2 * np.pi * 0.02 * 0.25) < 1e-15
+    b = oracle.OracleEngine(shape, 1.0, 0.0, 0.01, 100.0, ETF, HARMONIC)
+    b.set_user_params([0.0, 0.0, 0.0, 1.5])
+    for rho in (1e-3, 0.05, 0.8):  # dEDF/dn of the ETF functional is the derivative of its EDF
+        h = rho * 1e-5
+        num = (b.dedfdn(rho + h)[1] - b.dedfdn(rho - h)[1]) / (2 * h)
+        assert abs(num - b.dedfdn(rho)[0]) < 1e-7 * abs(b.dedfdn(rho)[0])
+    # Gamma: DIMERS for UFG/ETF, PARTICLES for contact/dipolar (dev/gpe_user_defined.h)
+    assert e.gamma == 2.0 and c.gamma == 1.0 and b.gamma == 2.0
+    ws = oracle.OracleEngine(shape, 1.0, 0.0, 0.01, 100.0, CONTACT, WOODS_SAXON)
+    ws.set_user_params([2.0, 0.5, 3.0, 0.0])
+    r = np.hypot(0 - 8, 0 - 6)
+    assert abs(ws.vext(0, 0, 3) - 2.0 * (1 - 1 / (1 + np.exp((r - 3.0) / 0.5)))) < 1e-14
+    cs = oracle.OracleEngine(shape, 1.0, 0.0, 0.01, 100.0, CONTACT, COSINE)
+    cs.set_user_params([0.1, 2.0, 0.0, 0.0])
+    assert abs(cs.vext(4, 3, 0) - 0.1 * (np.cos(np.pi) + np.cos(np.pi) + 1.0)) < 1e-15
+
+
+@pytest.mark.parametrize("gam,a", [(1, 0.0), (1, 0.013), (2, 0.0)])
+def test_free_plane_wave_exact_phase(oracle, gam, a):
+    # With V_ext = 0 and uniform density, one step multiplies psi by
+    # exp(-i dt (|k|^2/(2 gamma) + gamma * 4 pi a rho)) exactly (RTE).
+    shape = (8, 6, 4)
+    dt = 0.05
+    psi, k = plane_wave(shape, (1, -2, 1), 0.7)
+    e = oracle.OracleEngine(shape, 1.0, 0.0, dt, 1.0, CONTACT, HARMONIC, gam)
+    e.set_user_params([0.0, 0.0, 0.0, a])
+    e.set_psi(psi)
+    e.evolve_qf(3)
+    out, t = e.get_psi()
+    rho = gam * 0.49
+    ph = -dt * (sum(kk * kk for kk in k) / (2 * gam) + gam * 4 * np.pi * a * rho)
+    assert abs(t - 3 * dt) < 1e-15
+    assert np.max(np.abs(out - psi * np.exp(3j * ph))) < 1e-13
+
+
+def test_rte_unitary_and_energy_drift(oracle):
+    shape = (16, 12, 8)
+    rng = np.random.default_rng(11)
+    e = oracle.OracleEngine(shape, 1.0, 0.0, 0.01, 500.0, UFG, HARMONIC)
+    e.set_user_params([0.1, 0.12, 0.15])
+    psi = rng.standard_normal(shape) + 1j * rng.standard_normal(shape) + 2.0
+    e.set_psi(psi)
+    e.normalize_psi()
+    n0 = e.get_norm()
+    assert abs(n0 - 500.0) < 1e-10
+    _, ek0, ei0, ee0 = e.energy()
+    e.evolve_qf(20)
+    assert abs(e.get_norm() / n0 - 1) < 1e-13
+    _, ek1, ei1, ee1 = e.energy()
+    E0, E1 = ek0 + ei0 + ee0, ek1 + ei1 + ee1
+    assert abs(E1 / E0 - 1) < 2e-2  # bounded drift of a noisy start
+
+
+def test_ite_energy_monotone_and_norm(oracle):
+    shape = (16, 12, 8)
+    rng = np.random.default_rng(12)
+    e = oracle.OracleEngine(shape, 0.0, 1.0, 0.05, 200.0, UFG, HARMONIC)
+    e.set_user_params([0.1, 0.12, 0.15])
+    e.set_psi(rng.random(shape) + 0.5)
+    e.normalize_psi()
+    prev = np.inf
+    for _ in range(6):
+        mu = e.evolve_qf(10, want_mu=True)
+        assert abs(e.get_norm() - 200.0) < 1e-9
+        _, ek, ei, ee = e.energy()
+        E = ek + ei + ee
+        assert E < prev
+        prev = E
+    assert np.isfinite(mu)
+
+
+def lattice_ho_ground(n, omega, gam):
+    """Ground-state energy of the 1-D periodic spectral Hamiltonian
+    k^2/(2 gamma) + gamma * omega^2 (i - n/2)^2 / 2 by dense diagonalisation."""
+    k = 2 * np.pi * np.fft.fftfreq(n)
+    F = np.fft.fft(np.eye(n), axis=0)
+    T = (np.conj(F).T @ np.diag(k**2 / (2 * gam)) @ F).real / n
+    x = np.arange(n) - n // 2
+    H = T + np.diag(gam * 0.5 * omega**2 * x**2)
+    return np.linalg.eigvalsh(H)[0]
+
+
+def test_ite_converges_to_lattice_ho_ground_state(oracle):
+    # g = 0 in a harmonic trap: the ITE fixed point is the lattice ground state up
+    # to the O(dt^2) splitting error; both E/N and mu must match it.
+    shape = (16, 16, 12)
+    om = (0.35, 0.3, 0.45)
+    E0 = sum(lattice_ho_ground(n, w, 1.0) for n, w in zip(shape, om))
+    errs, merrs = [], []
+    for dt in (0.1, 0.05):
+        e = oracle.OracleEngine(shape, 0.0, 1.0, dt, 10.0, CONTACT, HARMONIC, 1)
+        e.set_user_params([om[0], om[1], om[2], 0.0])
+        e.set_psi(np.ones(shape, dtype=complex))
+        e.normalize_psi()
+        mu = e.evolve_qf(int(round(60 / dt)), want_mu=True)
+        _, ek, ei, ee = e.energy()
+        EN = (ek + ei + ee) / 10.0
+        errs.append(abs(EN - E0))
+        merrs.append(abs(mu - E0))
+        assert abs(EN / E0 - 1) < 1e-6
+        assert abs(mu / E0 - 1) < 1e-3
+    # second-order splitting: the state (and mu from the norm decay) carries an
+    # O(dt^2) error, the variational energy an O(dt^4) one
+    assert 3.5 < merrs[0] / merrs[1] < 4.5
+    assert 12.0 < errs[0] / errs[1] < 20.0
+
+
+def test_energy_closed_forms(oracle):
+    shape = (8, 6, 10)
+    gam = 2
+    psi, k = plane_wave(shape, (2, 1, -3), 0.3)
+    e = oracle.OracleEngine(shape, 0.0, 1.0, 0.01, 1.0, CONTACT, HARMONIC, gam)
+    e.set_user_params([0.2, 0.0, 0.5, 0.07])
+    e.set_psi(psi)
+    t, ek, ei, ee = e.energy()  # beta != 0: no renormalisation
+    N = np.prod(shape)
+    rho = gam * 0.09
+    assert abs(ek - N * 0.09 * sum(kk * kk for kk in k) / (2 * gam)) < 1e-12
+    assert abs(ei - N * 2 * np.pi * 0.07 * rho**2) < 1e-12
+    # sum_i (i - n/2)^2 over a periodic axis of even length n = n (n^2 + 2) / 12
+    s = lambda n: n * (n * n + 2) / 12
+    ref = rho * 0.5 * (0.04 * s(8) * 6 * 10 + 0.25 * s(10) * 8 * 6)
+    assert abs(ee - ref) < 1e-12 * ref
+    # RTE energy renormalises psi first (orginal/gpe_engine.cu:1084-1088)
+    r = oracle.OracleEngine(shape, 1.0, 0.0, 0.01, 3.0, CONTACT, HARMONIC, gam)
+    r.set_user_params([0.0, 0.0, 0.0, 0.0])
+    r.set_psi(psi)
+    _, ek2, _, _ = r.energy()
+    assert abs(r.get_norm() - 3.0) < 1e-12
+    assert abs(ek2 - 3.0 * sum(kk * kk for kk in k) / (2 * gam) / gam) < 1e-12
+
+
+def test_currents_and_density_plane_wave(oracle):
+    shape = (8, 6, 10)
+    gam = 2
+    psi, k = plane_wave(shape, (1, 2, -1), 0.5)
+    e = oracle.OracleEngine(shape, 1.0, 0.0, 0.01, 1.0, UFG, HARMONIC, gam)
+    e.set_psi(psi, t=1.25)
+    (jx, jy, jz), t = e.get_currents()
+    assert t == 1.25
+    for j, kk in zip((jx, jy, jz), k):
+        assert np.max(np.abs(j - 0.25 * kk / gam)) < 1e-14
+    rho, _ = e.get_density()
+    assert np.max(np.abs(rho - gam * 0.25)) < 1e-15
+
+
+def test_quantum_friction_vanishes_on_plane_wave_and_dissipates(oracle):
+    shape = (8, 6, 4)
+    psi, k = plane_wave(shape, (1, 0, 1), 0.6)
+    # uniform-density plane wave: div j = 0, V_qf = 0, so the exact phase survives
+    e = oracle.OracleEngine(shape, 1.0, 0.0, 0.05, 1.0, CONTACT, HARMONIC, 1)
+    e.set_user_params([0.0, 0.0, 0.0, 0.0])
+    e.set_quantum_friction_coeff(0.5)
+    e.set_psi(psi)
+    e.evolve_qf(2)
+    out, _ = e.get_psi()
+    ph = -0.05 * sum(kk * kk for kk in k) / 2
+    assert np.max(np.abs(out - psi * np.exp(2j * ph))) < 1e-12
+    # a non-stationary state loses energy under quantum friction at fixed norm
+    shape = (16, 12, 8)
+    rng = np.random.default_rng(5)
+    f = oracle.OracleEngine(shape, 1.0, 0.0, 0.02, 300.0, UFG, HARMONIC)
+    f.set_user_params([0.1, 0.12, 0.15])
+    f.set_quantum_friction_coeff(0.8)
+    f.set_psi(rng.standard_normal(shape) + 1j * rng.standard_normal(shape) + 1.5)
+    f.normalize_psi()
+    Es = []
+    for _ in range(4):
+        f.evolve_qf(10)
+        _, a, b, c = f.energy()
+        Es.append(a + b + c)
+    assert all(x > y for x, y in zip(Es, Es[1:]))
+    assert abs(f.get_norm() - 300.0) < 1e-8
+
+
+def test_vortex_imprint_phase_and_modulus(oracle):
+    # dev/gpe_engine.cu:485-511: phase Q * atan2(x, y), x = ix - nx/2 - x0, |psi| kept
+    shape = (12, 10, 4)
+    rng = np.random.default_rng(9)
+    amp = rng.random(shape) + 0.1
+    e = oracle.OracleEngine(shape, 0.0, 1.0, 1e-9, 1.0, CONTACT, HARMONIC, 1)
+    e.set_user_params([0.0, 0.0, 0.0, 0.0])
+    e.set_vortex(0.5, -1.5, 2)
+    e.set_psi(amp.astype(complex))
+    e.evolve_vortex(1)
+    out, _ = e.get_psi()
+    ix, iy = np.meshgrid(np.arange(12), np.arange(10), indexing="ij")
+    ph = 2 * np.arctan2(ix - 6 - 0.5, iy - 5 + 1.5)
+    # dt -> 0 with no potential: one vortex step is the imprint up to O(dt)
+    exp_phase = np.exp(1j * ph)[:, :, None]
+    assert np.max(np.abs(np.angle(out / (np.abs(out) * exp_phase)))) < 1e-6
