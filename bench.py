@@ -1,0 +1,215 @@
+"""Headline benchmark: real-time split steps of a 256^3 Bose gas in complex64.
+
+Workload (one "step" = one predictor-corrector split step of the reference's
+gpe_evolve_qf, orginal/gpe_engine.cu:948-1013): 256^3 lattice, contact bosons
+(gamma = 1, a = 0.008, one particle per site), cosine external potential of
+amplitude 0.1 and period L/8 on each axis, dt = 0.02, a seeded random-phase
+state with 20 z-line vortices of winding +-1.  The state is resident in HBM
+before the timed region starts.
+
+Prints one JSON line with the throughput, the HIP-event roofline of the
+dominant kernel (the strided y/x FFT pass), the energy error of complex64
+against the complex128 engine on the same steps, and the FP64 CPU oracle
+timed on this host as the CPU baseline.
+
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 50]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "split-steps/s at 256^3 c64 vs c128 (rel-E err≤1e-6), %HBM peak; ensemble steps/s"
+HBM_PEAK_GBS = 8000.0
+A_SCAT = 0.008
+DT = 0.02
+
+
+def initial_state(shape, seed=3, nvort=20):
+    """Seeded random-phase state with nvort z-line vortices of winding +-1."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = shape
+    ix, iy = np.meshgrid(np.arange(nx, dtype=np.float64), np.arange(ny, dtype=np.float64), indexing="ij")
+    ph = np.zeros((nx, ny))
+    x0 = rng.uniform(0, nx, nvort) + 0.5
+    y0 = rng.uniform(0, ny, nvort) + 0.5
+    w = rng.choice([-1.0, 1.0], nvort)
+    for j in range(nvort):
+        ph += w[j] * np.arctan2(iy - y0[j], ix - x0[j])
+    psi = np.empty(shape, dtype=np.complex128)
+    noise = rng.standard_normal((nx, ny, nz)).astype(np.float32)
+    amp = (1.0 + 0.01 * rng.random((nx, ny, nz))).astype(np.float32)
+    psi[...] = np.exp(1j * (ph[:, :, None] + 0.1 * noise)) * amp
+    return psi
+
+
+def make_engine(E, shape, precision, psi):
+    n = int(np.prod(shape))
+    eng = E.Engine(shape, 1.0, 0.0, DT, float(n), E.CONTACT, E.COSINE, 0, precision)
+    eng.set_user_params([0.1, 8.0, 0.0, A_SCAT])
+    eng.set_psi(psi)
+    eng.normalize_psi()
+    return eng
+
+
+def timed(eng, steps):
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.evolve_qf(steps)
+    eng.synchronize()
+    return time.perf_counter() - t0
+
+
+def pmc_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    --pmc summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel factor)."""
+    p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(p):
+        try:
+            return json.load(open(p)).get("bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+def cpu_baseline(shape, psi, steps):
+    from oracle import pyoracle
+
+    n = int(np.prod(shape))
+    o = pyoracle.OracleEngine(shape, 1.0, 0.0, DT, float(n), -1, 2, 0)
+    o.set_user_params([0.1, 8.0, 0.0, A_SCAT])
+    o.set_psi(psi)
+    o.normalize_psi()
+    o.evolve_qf(1)
+    t0 = time.perf_counter()
+    o.evolve_qf(steps)
+    dt = time.perf_counter() - t0
+    threads = pyoracle.lib().gpeo_num_threads()
+    o.close()
+    return {
+        "value": steps / dt,
+        "unit": "split-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "%d FP64 oracle steps of the same 256^3 workload (complex128, own FFT, %d OpenMP threads)"
+        % (steps, threads),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c128", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this engine runs on one GPU; a %d-GPU run is not part of this build "
+              "(the sharded design is described in DESIGN.md)" % a.gpus, file=sys.stderr)
+        return 2
+
+    from libcugpe_amd import engine as E
+
+    E.lib()
+    shape = (a.size,) * 3
+    n = int(np.prod(shape))
+    print("[bench] building the %s initial state" % (shape,), file=sys.stderr, flush=True)
+    psi = initial_state(shape)
+
+    print("[bench] complex64 engine: warmup %d, timed %d" % (a.warmup, a.steps), file=sys.stderr, flush=True)
+    eng = make_engine(E, shape, E.C64, psi)
+    eng.evolve_qf(a.warmup)
+    t = timed(eng, a.steps)
+    sps = a.steps / t
+
+    # same number of steps again with HIP events around every launch of the
+    # dominant kernel (strided FFT pass) and, separately, of the z pass
+    eng.profile(0, True)
+    t_prof = timed(eng, a.steps)
+    ms_strided, n_strided = eng.profile_read()
+    eng.profile(0, False)
+    eng.profile(1, True)
+    eng.evolve_qf(a.steps)
+    ms_z, n_z = eng.profile_read()
+    eng.profile(1, False)
+    total_steps = a.warmup + 3 * a.steps
+    _, ek, ei, ee = eng.energy()
+    e64 = ek + ei + ee
+    eng.close()
+
+    S = 8 * n  # bytes of one complex64 state array
+    avg_ms = ms_strided / max(n_strided, 1)
+    bytes_launch = 2 * S  # one read and one write of the state per axis pass
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    roofline = {
+        "kernel": "fft_pass_kernel<float, %d, strided> (y and x axis passes)" % a.size,
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": pmc_traffic(),
+        "avg_launch_ms": avg_ms,
+        "launches_timed": n_strided,
+        "algorithmic_bytes_per_launch": bytes_launch,
+    }
+    z_avg = ms_z / max(n_z, 1)
+
+    result = {
+        "metric": METRIC,
+        "value": sps,
+        "unit": "split-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * t / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {
+            "workload": "256^3 real-time contact BEC (a=%g, 1 particle/site), cosine V_ext A=0.1 period L/8, "
+            "20 z-line vortices, dt=%g, complex64 state, FP64 reductions" % (A_SCAT, DT),
+            "lattice": list(shape),
+            "precision": "complex64",
+        },
+        "step_hbm_frac": 22 * S * sps / (HBM_PEAK_GBS * 1e9),
+        "step_algorithmic_bytes": 22 * S,
+        "z_pass_avg_ms": z_avg,
+        "profiled_region_steps_per_s": a.steps / t_prof,
+        "roofline": roofline,
+    }
+
+    if not a.no_c128:
+        print("[bench] complex128 engine on the same steps", file=sys.stderr, flush=True)
+        e128 = make_engine(E, shape, E.C128, psi)
+        e128.evolve_qf(a.warmup)
+        t128 = timed(e128, a.steps)
+        e128.evolve_qf(total_steps - a.warmup - a.steps)
+        _, ek, ei, ee = e128.energy()
+        eref = ek + ei + ee
+        e128.close()
+        result["c128_steps_per_s"] = a.steps / t128
+        result["c128_step_hbm_frac"] = 22 * 2 * S * (a.steps / t128) / (HBM_PEAK_GBS * 1e9)
+        result["energy_rel_err_c64_vs_c128"] = abs(e64 - eref) / abs(eref)
+        result["energy_steps_compared"] = total_steps
+
+    if not a.no_cpu_baseline:
+        print("[bench] FP64 CPU oracle baseline (%d steps)" % a.cpu_steps, file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(shape, psi, a.cpu_steps)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,228 @@
+/*
+ * MI355X-native split-step Gross-Pitaevskii engine: the C ABI.
+ *
+ * Drop-in boundary for the reference's gpe_engine.h (dev/gpe_engine.h:74-558,
+ * orginal/gpe_engine.h:42-192).  Every function below is exported with C
+ * linkage from libcugpe_amd/libgpe_hip.so; arguments are plain pointers,
+ * sizes and doubles.  Two call surfaces share one implementation:
+ *
+ *  - the reference's singleton surface (gpe_create_engine ... gpe_destroy_engine),
+ *    same names, argument meaning, defaults and error behaviour.  The lattice
+ *    and physics the reference fixes at library build time (-DNX -DNY -DNZ
+ *    -DINTERACTIONS -DVEXT) are registered at program start by this header when
+ *    the driver is compiled with -DNX=.. -DNY=.. -DNZ=.., or by an explicit
+ *    gpe_configure_lattice() call;
+ *  - a handle-based surface (gpe_engine_*) for any number of engines of any
+ *    lattice, physics and precision in one process.
+ *
+ * Layout: psi is complex128 {double x, y} (cuDoubleComplex) in C order
+ * [nx][ny][nz], iz fastest.  Host buffers are owned by the caller and copied.
+ *
+ * Return codes (gpe_result_t): 0 success; > 0 a hipError_t from the HIP
+ * runtime; < 0 an engine error, see GPE_ERR_* (the reference returns raw
+ * cudaError_t / cufftResult values and -9 / -99 the same way).
+ */
+#ifndef GPE_ENGINE_H
+#define GPE_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "gpe_timing.h"
+#include "gpe_user_defined.h"
+
+#ifdef __cplusplus
+#define GPE_DEFAULT(v) = v
+extern "C" {
+#else
+#define GPE_DEFAULT(v)
+#endif
+
+typedef int gpe_result_t;
+typedef unsigned int uint;
+
+/* complex128 with the layout and alignment of cuDoubleComplex */
+typedef struct __attribute__((aligned(16))) {
+    double x, y;
+} cuCplx;
+typedef cuCplx Complex; /* orginal/gpe_engine.h:171 */
+
+#define GPE_SUCCESS 0
+#define GPE_ERR_TOO_MANY_PARAMS (-9) /* size > MAX_USER_PARAMS (reference code)      */
+#define GPE_ERR_NO_ENGINE (-10)      /* no engine created / NULL handle               */
+#define GPE_ERR_LATTICE (-11)        /* lattice not configured or length unsupported  */
+#define GPE_ERR_ARG (-12)            /* invalid argument                              */
+#define GPE_ERR_VORTEX_NOT_SET (-13) /* gpe_evolve_vortex before gpe_set_vortex       */
+#define GPE_ERR_PHASE_NOT_SET (-15)  /* enforced-phase evolution without a phase      */
+#define GPE_ERR_GAMMA (-99)          /* GPE_FOR not PARTICLES / DIMERS (reference code) */
+
+#define GPE_PRECISION_C128 128
+#define GPE_PRECISION_C64 64
+
+/* ------------------------------------------------------------------------
+ * Reference singleton surface (dev/gpe_engine.h:299-558).
+ * --------------------------------------------------------------------- */
+
+/* Lattice, physics and precision of the singleton engine; replaces the
+ * reference's library build flags (dev/Makefile:37-51).  gpe_for 0 selects
+ * the default for the interactions; precision GPE_PRECISION_C128 (default)
+ * or GPE_PRECISION_C64. */
+int gpe_configure_lattice(int nx, int ny, int nz, int interactions, int vext, int gpe_for, int precision);
+
+/* dev/gpe_engine.h:299 */
+void gpe_get_lattice(int *_nx, int *_ny, int *_nz);
+/* dev/gpe_engine.h:310; nthreads is accepted and ignored (launch shapes are
+ * the engine's own) */
+int gpe_create_engine(double alpha, double beta, double dt, double npart, int nthreads GPE_DEFAULT(1024));
+/* dev/gpe_engine.h:319 */
+int gpe_change_alpha_beta(double alpha, double beta);
+/* dev/gpe_engine.h:326, 333 */
+int gpe_set_rte_evolution(void);
+int gpe_set_ite_evolution(void);
+/* dev/gpe_engine.h:341 */
+int gpe_set_time(double t0);
+/* dev/gpe_engine.h:352; copies `size` values, the rest are zero */
+int gpe_set_user_params(int size, double *params);
+/* dev/gpe_engine.h:361 */
+int gpe_set_quantum_friction_coeff(double qfcoeff);
+/* dev/gpe_engine.h:370 */
+int gpe_set_vortex(double vortex_x0, double vortex_y0, int8_t Q GPE_DEFAULT(1));
+/* dev/gpe_engine.h:378, 385 */
+int gpe_get_phase(double *h_phase GPE_DEFAULT(NULL));
+int gpe_set_phase(double *h_phase);
+/* dev/gpe_engine.h:393, 402 */
+int gpe_set_psi(double t, cuCplx *psi);
+int gpe_get_psi(double *t, cuCplx *psi);
+/* dev/gpe_engine.h:412, 425 */
+int gpe_get_density(double *t, double *density);
+int gpe_get_currents(double *t, double *jx, double *jy, double *jz);
+/* dev/gpe_engine.h:432, 439 */
+int gpe_normalize_psi(void);
+int gpe_get_norm(double *norm);
+/* dev/gpe_engine.h:446: evolves nt split steps (the quantum-friction
+ * potential is applied when its coefficient is non-zero) */
+int gpe_evolve(int nt);
+/* dev/gpe_engine.h:455 */
+int gpe_evolve_qf(int nt, double *chemical_potential GPE_DEFAULT(NULL));
+/* dev/gpe_engine.h:465 */
+int gpe_evolve_vortex(int nt, double *chemical_potential GPE_DEFAULT(NULL));
+/* dev/gpe_engine.h:545 */
+int gpe_energy(double *t, double *ekin, double *eint, double *eext);
+/* dev/gpe_engine.h:552 */
+int gpe_destroy_engine(void);
+/* dev/gpe_engine.h:558 */
+void gpe_print_potential_type(void);
+/* dev/gpe_engine.h:158-195 */
+const char *gpe_get_error_string(gpe_result_t res);
+
+/* External potential array for VEXT 3 (no counterpart in the reference,
+ * whose potentials are compiled in). */
+int gpe_set_external_potential(const double *v);
+/* Plaquette vortex detection on the current psi (no counterpart in the
+ * reference; replaces its host tracker dev/gpe_vortex_detect.h:153-174):
+ * windings[3][N] (NULL allowed), number of non-zero windings in *count. */
+int gpe_detect_vortices(double rho_min, int8_t *windings, long *count);
+
+/* ------------------------------------------------------------------------
+ * Handle-based surface.
+ * --------------------------------------------------------------------- */
+
+typedef struct gpe_engine_s *gpe_engine_t;
+
+typedef struct {
+    int nx, ny, nz;
+    int interactions; /* INTERACTIONS code */
+    int vext;         /* VEXT code */
+    int gpe_for;      /* PARTICLES, DIMERS or 0 for the default */
+    int precision;    /* GPE_PRECISION_C128 or GPE_PRECISION_C64 */
+} gpe_config_t;
+
+int gpe_engine_create(gpe_engine_t *out, const gpe_config_t *cfg, double alpha, double beta, double dt, double npart);
+int gpe_engine_destroy(gpe_engine_t e);
+int gpe_engine_change_alpha_beta(gpe_engine_t e, double alpha, double beta);
+int gpe_engine_set_time(gpe_engine_t e, double t0);
+int gpe_engine_set_user_params(gpe_engine_t e, int size, const double *params);
+int gpe_engine_set_quantum_friction_coeff(gpe_engine_t e, double qfcoeff);
+int gpe_engine_set_vortex(gpe_engine_t e, double x0, double y0, int Q);
+int gpe_engine_set_external_potential(gpe_engine_t e, const double *v);
+int gpe_engine_set_psi(gpe_engine_t e, double t, const cuCplx *psi);
+int gpe_engine_get_psi(gpe_engine_t e, double *t, cuCplx *psi);
+int gpe_engine_normalize_psi(gpe_engine_t e);
+int gpe_engine_get_norm(gpe_engine_t e, double *norm);
+int gpe_engine_evolve_qf(gpe_engine_t e, int nt, double *chemical_potential);
+int gpe_engine_evolve_vortex(gpe_engine_t e, int nt, double *chemical_potential);
+int gpe_engine_energy(gpe_engine_t e, double *t, double *ekin, double *eint, double *eext);
+int gpe_engine_get_density(gpe_engine_t e, double *t, double *density);
+int gpe_engine_get_currents(gpe_engine_t e, double *t, double *jx, double *jy, double *jz);
+int gpe_engine_get_phase(gpe_engine_t e, double *h_phase);
+int gpe_engine_set_phase(gpe_engine_t e, const double *h_phase);
+int gpe_engine_detect_vortices(gpe_engine_t e, double rho_min, int8_t *windings, long *count);
+int gpe_engine_get_time(gpe_engine_t e, double *t);
+/* Unnormalised 3-D DFT of psi in place, sign -1 forward / +1 inverse, by the
+ * engine's own FFT passes (no counterpart in the reference, whose transform
+ * is cuFFT's; exposed so the transform can be checked on its own). */
+int gpe_engine_fft(gpe_engine_t e, int sign);
+int gpe_engine_get_config(gpe_engine_t e, gpe_config_t *cfg);
+/* device address of psi (complex64 or complex128 per the precision) and the
+ * HIP stream the engine launches on, for interop and timing */
+int gpe_engine_device_psi(gpe_engine_t e, void **ptr);
+int gpe_engine_stream(gpe_engine_t e, void **stream);
+int gpe_engine_synchronize(gpe_engine_t e);
+/* Kernel timing with HIP events on the engine's stream: while enabled, every
+ * FFT pass launched on the selected axis class (0: strided y/x passes,
+ * 1: contiguous z passes) is bracketed by an event pair; read returns the
+ * summed milliseconds and the number of timed launches. */
+int gpe_engine_profile(gpe_engine_t e, int axis_class, int enable);
+int gpe_engine_profile_read(gpe_engine_t e, double *total_ms, long *launches);
+/* library build information (GPU target, compiled FFT lengths) */
+const char *gpe_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+/* ------------------------------------------------------------------------
+ * Driver-side helpers of the reference header (dev/gpe_engine.h:119-133,
+ * 252-273): index macros, checked malloc and the abort-on-error wrapper.
+ * --------------------------------------------------------------------- */
+#define ixyz2ixiyiz(ixyz, _ix, _iy, _iz, i) \
+    i = ixyz;                               \
+    _ix = i / (ny * nz);                    \
+    i = i - _ix * ny * nz;                  \
+    _iy = i / nz;                           \
+    _iz = i - _iy * nz;
+
+#define ixiyiz2ixyz(_ixyz, _ix, _iy, _iz, ny, nz) _ixyz = _iz + nz * _iy + nz * ny * _ix
+
+#define gpemalloc(pointer, size, type)                                              \
+    if ((pointer = (type *)malloc((size) * sizeof(type))) == NULL) {                \
+        fprintf(stderr, "error: cannot malloc()! file=`%s`, line=%d\n", __FILE__, __LINE__); \
+        exit(EXIT_FAILURE);                                                         \
+    }
+
+#define gpe_exec(cmd, ierr)                                                                      \
+    {                                                                                            \
+        ierr = cmd;                                                                              \
+        if (ierr) {                                                                              \
+            fprintf(stderr, "error: cannot execute: %s\tfile=`%s`, line=%d\t", #cmd, __FILE__, __LINE__); \
+            fprintf(stderr, "Error=%d (%s)\nExiting!\n", ierr, gpe_get_error_string(ierr));     \
+            exit(EXIT_FAILURE);                                                                  \
+        }                                                                                        \
+    }
+
+/* A driver compiled with -DNX=.. -DNY=.. -DNZ=.. (the reference's build flags)
+ * registers its lattice and physics before main(), so the unchanged
+ * gpe_create_engine(alpha, beta, dt, npart) call creates that engine. */
+#if defined(NX) && defined(NY) && defined(NZ)
+#ifndef GPE_PRECISION
+#define GPE_PRECISION GPE_PRECISION_C128
+#endif
+static void __attribute__((constructor, used)) gpe_register_compiled_lattice_(void)
+{
+    gpe_configure_lattice(NX, NY, NZ, INTERACTIONS, VEXT, GPE_FOR, GPE_PRECISION);
+}
+#endif
+
+#endif

@@ -1,0 +1,70 @@
+// Launchers of the engine's device kernels.  Each precision is compiled in its
+// own translation unit (gpe_kernels_f32.hip, gpe_kernels_f64.hip).
+#pragma once
+
+#include "gpe_fft.hpp"
+
+namespace gpe {
+
+// lattice lengths with a compiled FFT pass
+bool fft_length_supported(int N);
+// pencils per workgroup for a compiled length
+template <typename T>
+int fft_tile_width(int N);
+
+// one FFT axis pass; returns the number of workgroups launched (0 on error)
+template <typename T>
+unsigned launch_fft_pass(int N, const PassArgs<T>& a, hipStream_t s);
+// the same for one compiled length (one object file per precision and length)
+template <typename T, int N>
+unsigned launch_fft_pass_n(const PassArgs<T>& a, hipStream_t s);
+
+#define GPE_FFT_LENGTHS(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(512)
+#define GPE_DECLARE_PASS(NN)                                                        \
+    template <>                                                                     \
+    unsigned launch_fft_pass_n<float, NN>(const PassArgs<float>& a, hipStream_t s); \
+    template <>                                                                     \
+    unsigned launch_fft_pass_n<double, NN>(const PassArgs<double>& a, hipStream_t s);
+GPE_FFT_LENGTHS(GPE_DECLARE_PASS)
+#undef GPE_DECLARE_PASS
+
+enum FinalizeMode {
+    FIN_NORM = 0,   // scal[0] = sum, scal[1] = sqrt(npart/sum), scal[2] = -log(sum/npart)/(2 dt)
+    FIN_STORE = 1,  // scal[slot + q] = sum_q / div
+};
+
+void launch_finalize(const double* partials, int nb, int nq, double* scal, int mode, int slot, double npart,
+                     double dt, double div, hipStream_t s);
+
+// sum of gamma |psi|^2 over the lattice into partials[0..nb), nb returned
+template <typename T>
+int launch_norm_partials(const cplx<T>* A, unsigned n, double gamma, double* partials, hipStream_t s);
+
+// A *= scal[1]
+template <typename T>
+void launch_scale(cplx<T>* A, unsigned n, const double* scal, hipStream_t s);
+
+// out = gamma |psi|^2 (FP64)
+template <typename T>
+void launch_density(const cplx<T>* A, unsigned n, double gamma, double* out, hipStream_t s);
+
+// plaquette windings w[3][n]; count of non-zero windings summed into partials, nb returned
+template <typename T>
+int launch_windings(const cplx<T>* A, int nx, int ny, int nz, double rho_min, int8_t* w, double* partials,
+                    hipStream_t s);
+
+// phase (arg psi) into a FP64 array, and enforcing a stored phase: psi = |psi| exp(i phase)
+template <typename T>
+void launch_phase(const cplx<T>* A, unsigned n, double* out, hipStream_t s);
+template <typename T>
+void launch_enforce_phase(cplx<T>* A, unsigned n, const double* phase, hipStream_t s);
+
+// complex128 <-> working precision
+template <typename T>
+void launch_from_c128(const cplx<double>* in, cplx<T>* out, unsigned n, hipStream_t s);
+template <typename T>
+void launch_to_c128(const cplx<T>* in, cplx<double>* out, unsigned n, hipStream_t s);
+
+constexpr int kMaxReduceBlocks = 1024;
+
+}  // namespace gpe

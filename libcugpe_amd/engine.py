@@ -1,0 +1,260 @@
+"""Python binding of the engine's C ABI (include/gpe_engine.h) over ctypes.
+
+`Engine` mirrors the reference's gpe_engine.h call surface
+(dev/gpe_engine.h:299-558) on a handle: same method names without the gpe_
+prefix, same argument meaning, and every non-zero return code raised as
+`GpeError` carrying the code and gpe_get_error_string().  The compute runs in
+libcugpe_amd/libgpe_hip.so; if that library is missing this module raises
+instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpe_hip.so")
+
+# physics codes (include/gpe_user_defined.h)
+UFG, ETF, CONTACT, DIPOLAR = 0, 1, -1, -2
+HARMONIC, WOODS_SAXON, COSINE, VARRAY = 0, 1, 2, 3
+PARTICLES, DIMERS = 1, 2
+C128, C64 = 128, 64
+
+_lib = None
+
+
+class GpeError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        super().__init__("%s failed: error %d (%s)" % (where, code, error_string(code)))
+
+
+class Config(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("nx", "ny", "nz", "interactions", "vext", "gpe_for", "precision")]
+
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_p = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    """Load libgpe_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError("libgpe_hip.so not built: run `python -m libcugpe_amd.build` (%s)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "gpe_engine_create": [ctypes.POINTER(_p), ctypes.POINTER(Config), _d, _d, _d, _d],
+            "gpe_engine_destroy": [_p],
+            "gpe_engine_change_alpha_beta": [_p, _d, _d],
+            "gpe_engine_set_time": [_p, _d],
+            "gpe_engine_set_user_params": [_p, _i, _p],
+            "gpe_engine_set_quantum_friction_coeff": [_p, _d],
+            "gpe_engine_set_vortex": [_p, _d, _d, _i],
+            "gpe_engine_set_external_potential": [_p, _p],
+            "gpe_engine_set_psi": [_p, _d, _p],
+            "gpe_engine_get_psi": [_p, _dp, _p],
+            "gpe_engine_normalize_psi": [_p],
+            "gpe_engine_get_norm": [_p, _dp],
+            "gpe_engine_evolve_qf": [_p, _i, _dp],
+            "gpe_engine_evolve_vortex": [_p, _i, _dp],
+            "gpe_engine_energy": [_p, _dp, _dp, _dp, _dp],
+            "gpe_engine_get_density": [_p, _dp, _p],
+            "gpe_engine_get_currents": [_p, _dp, _p, _p, _p],
+            "gpe_engine_get_phase": [_p, _p],
+            "gpe_engine_set_phase": [_p, _p],
+            "gpe_engine_detect_vortices": [_p, _d, _p, ctypes.POINTER(ctypes.c_long)],
+            "gpe_engine_get_time": [_p, _dp],
+            "gpe_engine_fft": [_p, _i],
+            "gpe_engine_get_config": [_p, ctypes.POINTER(Config)],
+            "gpe_engine_device_psi": [_p, ctypes.POINTER(_p)],
+            "gpe_engine_stream": [_p, ctypes.POINTER(_p)],
+            "gpe_engine_synchronize": [_p],
+            "gpe_engine_profile": [_p, _i, _i],
+            "gpe_engine_profile_read": [_p, _dp, ctypes.POINTER(ctypes.c_long)],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _i
+        L.gpe_get_error_string.argtypes = [_i]
+        L.gpe_get_error_string.restype = ctypes.c_char_p
+        L.gpe_build_info.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def error_string(code):
+    try:
+        return lib().gpe_get_error_string(code).decode()
+    except OSError:
+        return "library not loaded"
+
+
+def _check(rc, where):
+    if rc != 0:
+        raise GpeError(rc, where)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Engine:
+    """One engine instance (handle) of a given lattice, physics and precision."""
+
+    def __init__(self, shape, alpha, beta, dt, npart, interactions=UFG, vext=HARMONIC, gpe_for=0, precision=C128):
+        self.shape = tuple(int(s) for s in shape)
+        self.n = int(np.prod(self.shape))
+        self.precision = precision
+        cfg = Config(self.shape[0], self.shape[1], self.shape[2], interactions, vext, gpe_for, precision)
+        h = ctypes.c_void_p()
+        _check(lib().gpe_engine_create(ctypes.byref(h), ctypes.byref(cfg), alpha, beta, dt, npart), "gpe_engine_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gpe_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------- setters
+    def change_alpha_beta(self, alpha, beta):
+        _check(lib().gpe_engine_change_alpha_beta(self._h, alpha, beta), "gpe_change_alpha_beta")
+
+    def set_rte_evolution(self):
+        self.change_alpha_beta(1.0, 0.0)
+
+    def set_ite_evolution(self):
+        self.change_alpha_beta(0.0, 1.0)
+
+    def set_time(self, t0):
+        _check(lib().gpe_engine_set_time(self._h, t0), "gpe_set_time")
+
+    def set_user_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        _check(lib().gpe_engine_set_user_params(self._h, p.size, _ptr(p)), "gpe_set_user_params")
+
+    def set_quantum_friction_coeff(self, qf):
+        _check(lib().gpe_engine_set_quantum_friction_coeff(self._h, qf), "gpe_set_quantum_friction_coeff")
+
+    def set_vortex(self, x0, y0, Q=1):
+        _check(lib().gpe_engine_set_vortex(self._h, x0, y0, int(Q)), "gpe_set_vortex")
+
+    def set_external_potential(self, v):
+        a = np.ascontiguousarray(v, dtype=np.float64).reshape(self.shape)
+        _check(lib().gpe_engine_set_external_potential(self._h, _ptr(a)), "gpe_set_external_potential")
+
+    # ------------------------------------------------------ wave function
+    def set_psi(self, psi, t=0.0):
+        a = np.ascontiguousarray(psi, dtype=np.complex128).reshape(self.shape)
+        _check(lib().gpe_engine_set_psi(self._h, t, _ptr(a)), "gpe_set_psi")
+
+    def get_psi(self):
+        a = np.empty(self.shape, dtype=np.complex128)
+        t = ctypes.c_double()
+        _check(lib().gpe_engine_get_psi(self._h, ctypes.byref(t), _ptr(a)), "gpe_get_psi")
+        return a, t.value
+
+    def normalize_psi(self):
+        _check(lib().gpe_engine_normalize_psi(self._h), "gpe_normalize_psi")
+
+    def get_norm(self):
+        v = ctypes.c_double()
+        _check(lib().gpe_engine_get_norm(self._h, ctypes.byref(v)), "gpe_get_norm")
+        return v.value
+
+    # ---------------------------------------------------------- evolution
+    def evolve_qf(self, nt, want_mu=False):
+        mu = ctypes.c_double(float("nan"))
+        _check(lib().gpe_engine_evolve_qf(self._h, nt, ctypes.byref(mu) if want_mu else None), "gpe_evolve_qf")
+        return mu.value if want_mu else None
+
+    evolve = evolve_qf
+
+    def evolve_vortex(self, nt, want_mu=False):
+        mu = ctypes.c_double(float("nan"))
+        _check(lib().gpe_engine_evolve_vortex(self._h, nt, ctypes.byref(mu) if want_mu else None), "gpe_evolve_vortex")
+        return mu.value if want_mu else None
+
+    # -------------------------------------------------------- observables
+    def energy(self):
+        t, ek, ei, ee = (ctypes.c_double() for _ in range(4))
+        _check(lib().gpe_engine_energy(self._h, ctypes.byref(t), ctypes.byref(ek), ctypes.byref(ei), ctypes.byref(ee)),
+               "gpe_energy")
+        return t.value, ek.value, ei.value, ee.value
+
+    def get_density(self):
+        a = np.empty(self.shape, dtype=np.float64)
+        t = ctypes.c_double()
+        _check(lib().gpe_engine_get_density(self._h, ctypes.byref(t), _ptr(a)), "gpe_get_density")
+        return a, t.value
+
+    def get_currents(self):
+        js = [np.empty(self.shape, dtype=np.float64) for _ in range(3)]
+        t = ctypes.c_double()
+        _check(lib().gpe_engine_get_currents(self._h, ctypes.byref(t), *(_ptr(j) for j in js)), "gpe_get_currents")
+        return js, t.value
+
+    def get_phase(self):
+        a = np.empty(self.shape, dtype=np.float64)
+        _check(lib().gpe_engine_get_phase(self._h, _ptr(a)), "gpe_get_phase")
+        return a
+
+    def set_phase(self, phase):
+        a = np.ascontiguousarray(phase, dtype=np.float64).reshape(self.shape)
+        _check(lib().gpe_engine_set_phase(self._h, _ptr(a)), "gpe_set_phase")
+
+    def detect_vortices(self, rho_min, want_windings=True):
+        w = np.zeros((3,) + self.shape, dtype=np.int8) if want_windings else None
+        cnt = ctypes.c_long()
+        _check(lib().gpe_engine_detect_vortices(self._h, rho_min, _ptr(w) if w is not None else None,
+                                                ctypes.byref(cnt)), "gpe_detect_vortices")
+        return w, cnt.value
+
+    def fft(self, sign):
+        """In-place unnormalised 3-D DFT of psi (sign -1 forward)."""
+        _check(lib().gpe_engine_fft(self._h, sign), "gpe_engine_fft")
+
+    def get_time(self):
+        t = ctypes.c_double()
+        _check(lib().gpe_engine_get_time(self._h, ctypes.byref(t)), "gpe_get_time")
+        return t.value
+
+    # -------------------------------------------------------- interop
+    def stream(self):
+        s = ctypes.c_void_p()
+        _check(lib().gpe_engine_stream(self._h, ctypes.byref(s)), "gpe_engine_stream")
+        return s.value
+
+    def device_psi(self):
+        p = ctypes.c_void_p()
+        _check(lib().gpe_engine_device_psi(self._h, ctypes.byref(p)), "gpe_engine_device_psi")
+        return p.value
+
+    def synchronize(self):
+        _check(lib().gpe_engine_synchronize(self._h), "gpe_engine_synchronize")
+
+    def profile(self, axis_class, enable=True):
+        _check(lib().gpe_engine_profile(self._h, axis_class, 1 if enable else 0), "gpe_engine_profile")
+
+    def profile_read(self):
+        ms = ctypes.c_double()
+        cnt = ctypes.c_long()
+        _check(lib().gpe_engine_profile_read(self._h, ctypes.byref(ms), ctypes.byref(cnt)), "gpe_engine_profile_read")
+        return ms.value, cnt.value

@@ -1,0 +1,231 @@
+"""GPU parity: the HIP engine through its C ABI against the FP64 CPU oracle on
+the same seeded inputs (complex128 to 1e-12 relative, complex64 to the
+tolerances stated per test), plus closed forms on the GPU."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from libcugpe_amd import engine as E  # noqa: E402
+
+TOL128 = 1e-12
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def seeded_state(shape, seed, phase=True):
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = shape
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    r2 = ((ix - nx / 2) / (nx / 4)) ** 2 + ((iy - ny / 2) / (ny / 4)) ** 2 + ((iz - nz / 2) / (nz / 4)) ** 2
+    amp = np.exp(-r2) * (1 + 0.05 * rng.random(shape))
+    ph = 0.3 * rng.standard_normal(shape) if phase else 0.0
+    return (amp * np.exp(1j * ph)).astype(np.complex128)
+
+
+def pair(oracle, shape, alpha, beta, dt, npart, interactions, vext, params, precision=E.C128, gpe_for=0):
+    g = E.Engine(shape, alpha, beta, dt, npart, interactions, vext, gpe_for, precision)
+    o = oracle.OracleEngine(shape, alpha, beta, dt, npart, interactions, vext, gpe_for)
+    g.set_user_params(params)
+    o.set_user_params(params)
+    return g, o
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (256, 32, 32), (24, 16, 12), (48, 8, 64), (16, 96, 8)])
+@pytest.mark.parametrize("prec", [E.C128, E.C64])
+def test_fft3d_matches_numpy(shape, prec):
+    say("fft3d", shape, prec)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    g = E.Engine(shape, 1.0, 0.0, 0.01, 1.0, precision=prec)
+    g.set_psi(x)
+    g.fft(-1)
+    f, _ = g.get_psi()
+    tol = 1e-14 if prec == E.C128 else 3e-6
+    assert rel_err(f, np.fft.fftn(x)) < tol
+    g.fft(+1)
+    b, _ = g.get_psi()
+    assert rel_err(b / x.size, x) < tol
+    g.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(E.C128, 1e-13), (E.C64, 2e-6)])
+def test_plane_wave_exact_phase(prec, tol):
+    say("plane wave", prec)
+    shape = (32, 16, 24)
+    nx, ny, nz = shape
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    k = (2 * np.pi * 3 / nx, -2 * np.pi * 2 / ny, 2 * np.pi * 1 / nz)
+    psi = 0.7 * np.exp(1j * (k[0] * ix + k[1] * iy + k[2] * iz))
+    dt, a = 0.05, 0.013
+    g = E.Engine(shape, 1.0, 0.0, dt, 1.0, E.CONTACT, E.HARMONIC, 0, prec)
+    g.set_user_params([0.0, 0.0, 0.0, a])
+    g.set_psi(psi)
+    g.evolve_qf(4)
+    out, t = g.get_psi()
+    ph = -dt * (sum(x * x for x in k) / 2 + 4 * np.pi * a * 0.49)
+    assert abs(t - 4 * dt) < 1e-15
+    assert rel_err(out, psi * np.exp(4j * ph)) < tol
+    g.close()
+
+
+CASES = [
+    # name, shape, alpha, beta, interactions, vext, params, gpe_for
+    ("ufg_harmonic", (32, 32, 32), 0.1, [0.1, 0.12, 0.15], 0, 0, 0),
+    ("compare_lattice", (256, 32, 32), 0.025, [0.01, 0.10, 0.11], 0, 0, 0),
+    ("contact_cosine", (24, 16, 12), 0.02, [0.1, 2.0, 0.0, 0.02], -1, 2, 0),
+    ("etf_woods_saxon", (48, 8, 64), 0.02, [1.0, 0.5, 6.0, 1.5], 1, 1, 0),
+    ("ufg_particles", (16, 96, 8), 0.03, [0.2, 0.05, 0.3], 0, 0, 1),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("mode", ["rte", "ite"])
+def test_evolve_matches_oracle(oracle, case, mode):
+    name, shape, dt, params, inter, vext, gfor = case
+    alpha, beta = (1.0, 0.0) if mode == "rte" else (0.0, 1.0)
+    say("evolve", name, mode)
+    g, o = pair(oracle, shape, alpha, beta, dt, 800.0, inter, vext, params, gpe_for=gfor)
+    psi = seeded_state(shape, 7)
+    for x in (g, o):
+        x.set_psi(psi)
+        x.normalize_psi()
+    for nsteps in (1, 9):
+        mg = g.evolve_qf(nsteps, want_mu=True)
+        mo = o.evolve_qf(nsteps, want_mu=True)
+        pg, tg = g.get_psi()
+        po, to = o.get_psi()
+        assert tg == to
+        assert rel_err(pg, po) < TOL128, (name, mode, nsteps)
+        if mode == "ite":
+            assert abs(mg - mo) <= TOL128 * max(1.0, abs(mo))
+    eg, eo = g.energy(), o.energy()
+    for a, b in zip(eg[1:], eo[1:]):
+        assert abs(a - b) <= TOL128 * max(abs(b), 1e-3 * sum(abs(x) for x in eo[1:]))
+    assert abs(g.get_norm() - o.get_norm()) < TOL128 * 800.0
+    g.close()
+
+
+@pytest.mark.parametrize("mode", ["rte", "ite", "complex"])
+def test_quantum_friction_matches_oracle(oracle, mode):
+    alpha, beta = {"rte": (1.0, 0.0), "ite": (0.0, 1.0), "complex": (1.0, 0.1)}[mode]
+    say("qf", mode)
+    shape = (32, 24, 16)
+    g, o = pair(oracle, shape, alpha, beta, 0.02, 500.0, 0, 0, [0.1, 0.12, 0.15])
+    psi = seeded_state(shape, 11)
+    for x in (g, o):
+        x.set_quantum_friction_coeff(0.3)
+        x.set_psi(psi)
+        x.normalize_psi()
+    g.evolve_qf(6)
+    o.evolve_qf(6)
+    assert rel_err(g.get_psi()[0], o.get_psi()[0]) < TOL128
+    # switching quantum friction off mid-run
+    for x in (g, o):
+        x.set_quantum_friction_coeff(0.0)
+    g.evolve_qf(2)
+    o.evolve_qf(2)
+    assert rel_err(g.get_psi()[0], o.get_psi()[0]) < TOL128
+    g.close()
+
+
+@pytest.mark.parametrize("mode", ["rte", "ite"])
+def test_vortex_evolution_matches_oracle(oracle, mode):
+    alpha, beta = (1.0, 0.0) if mode == "rte" else (0.0, 1.0)
+    say("vortex", mode)
+    shape = (32, 32, 16)
+    g, o = pair(oracle, shape, alpha, beta, 0.025, 1000.0, 0, 0, [0.05, 0.05, 0.1])
+    psi = seeded_state(shape, 3, phase=False)
+    for x in (g, o):
+        x.set_psi(psi)
+        x.normalize_psi()
+        x.set_vortex(0.5, 3.5, 1)
+    mg = g.evolve_vortex(7, want_mu=True)
+    mo = o.evolve_vortex(7, want_mu=True)
+    assert rel_err(g.get_psi()[0], o.get_psi()[0]) < TOL128
+    if mode == "ite":
+        assert abs(mg - mo) < TOL128 * abs(mo)
+    g.close()
+
+
+def test_vortex_requires_set_vortex():
+    g = E.Engine((16, 16, 16), 0.0, 1.0, 0.01, 1.0)
+    with pytest.raises(E.GpeError) as ex:
+        g.evolve_vortex(1)
+    assert ex.value.code == -13
+    g.close()
+
+
+def test_observables_match_oracle(oracle):
+    say("observables")
+    shape = (24, 32, 16)
+    g, o = pair(oracle, shape, 1.0, 0.0, 0.01, 300.0, 0, 0, [0.1, 0.12, 0.15])
+    psi = seeded_state(shape, 5)
+    for x in (g, o):
+        x.set_psi(psi, t=2.5)
+    # density and currents (orginal/gpe_engine.cu:1115-1212)
+    dg, tg = g.get_density()
+    do, to = o.get_density()
+    assert tg == to == 2.5
+    assert rel_err(dg, do) < 1e-15
+    jg, _ = g.get_currents()
+    jo, _ = o.get_currents()
+    for a, b in zip(jg, jo):
+        assert rel_err(a, b) < TOL128
+    # energy renormalises in real time (orginal/gpe_engine.cu:1084-1088)
+    eg, eo = g.energy(), o.energy()
+    for a, b in zip(eg, eo):
+        assert abs(a - b) <= TOL128 * max(abs(b), 1.0)
+    assert rel_err(g.get_psi()[0], o.get_psi()[0]) < TOL128
+    assert abs(g.get_norm() - 300.0) < 1e-9
+    g.close()
+
+
+@pytest.mark.parametrize("prec", [E.C128, E.C64])
+def test_windings_bit_exact(oracle, prec):
+    say("windings", prec)
+    rng = np.random.default_rng(8)
+    shape = (32, 24, 16)
+    psi = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    g = E.Engine(shape, 1.0, 0.0, 0.01, 1.0, precision=prec)
+    g.set_psi(psi)
+    wg, cg = g.detect_vortices(0.05)
+    on_device = g.get_psi()[0]
+    ref_in = on_device.astype(np.complex64) if prec == E.C64 else on_device
+    wo, co = oracle.plaquette_windings(ref_in, 0.05)
+    assert cg == co and cg > 0
+    assert np.array_equal(wg, wo)
+    g.close()
+
+
+def test_c64_energy_error_vs_fp64(oracle):
+    """complex64 engine vs FP64 oracle after 200 real-time steps on 64^3."""
+    say("c64 accuracy")
+    shape = (64, 64, 64)
+    params = [0.1, 2.0, 0.0, 0.008]
+    g, o = pair(oracle, shape, 1.0, 0.0, 0.02, float(np.prod(shape)), E.CONTACT, E.COSINE, params, precision=E.C64)
+    rng = np.random.default_rng(2)
+    psi = np.exp(1j * 0.5 * rng.standard_normal(shape)) * (1 + 0.01 * rng.random(shape))
+    for x in (g, o):
+        x.set_psi(psi)
+        x.normalize_psi()
+    g.evolve_qf(200)
+    o.evolve_qf(200)
+    eg, eo = g.energy(), o.energy()
+    Eg, Eo = sum(eg[1:]), sum(eo[1:])
+    rel = abs(Eg - Eo) / abs(Eo)
+    say("c64 rel energy error after 200 steps: %.3e" % rel)
+    assert rel < 1e-6
+    dg, _ = g.get_density()
+    do, _ = o.get_density()
+    assert np.max(np.abs(dg - do)) / np.max(do) < 1e-3
+    g.close()
