@@ -48,6 +48,9 @@ typedef struct __attribute__((aligned(16))) {
     double x, y;
 } cuCplx;
 typedef cuCplx Complex; /* orginal/gpe_engine.h:171 */
+#ifndef GPE_NO_CPLX
+typedef double _Complex cplx; /* C99 complex128, dev/gpe_complex.cuh:36 */
+#endif
 
 #define GPE_SUCCESS 0
 #define GPE_ERR_TOO_MANY_PARAMS (-9) /* size > MAX_USER_PARAMS (reference code)      */
@@ -116,6 +119,14 @@ int gpe_destroy_engine(void);
 void gpe_print_potential_type(void);
 /* dev/gpe_engine.h:158-195 */
 const char *gpe_get_error_string(gpe_result_t res);
+
+/* Device selection and page-locked host memory for psi buffers: the drivers'
+ * cudaSetDevice / cudaHostAlloc / cudaFreeHost, served by the library so a
+ * driver needs no GPU runtime header (no counterpart in the reference's
+ * engine). */
+int gpe_set_device(int device);
+int gpe_host_alloc(void **ptr, size_t bytes);
+int gpe_host_free(void *ptr);
 
 /* External potential array for VEXT 3 (no counterpart in the reference,
  * whose potentials are compiled in). */

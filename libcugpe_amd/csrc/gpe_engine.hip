@@ -29,6 +29,7 @@
 #include <string>
 #include <vector>
 
+#define GPE_NO_CPLX  // the engine has its own complex template gpe::cplx
 #include "gpe_engine.h"
 #include "gpe_kernels.hpp"
 
@@ -64,6 +65,7 @@ struct gpe_engine_s {
     void *A = nullptr, *B = nullptr, *C = nullptr;  // complex state arrays
     void *Vb = nullptr, *Vq = nullptr, *varr = nullptr;  // real arrays
     void *tw[3] = {nullptr, nullptr, nullptr};        // per-axis twiddles
+    void *twlo[3] = {nullptr, nullptr, nullptr};      // complex64: low parts of the twiddles
     void *Ktab = nullptr, *ktab = nullptr;           // kinetic factors / wave numbers
     double *k64 = nullptr;
     double *partials = nullptr;
@@ -140,13 +142,13 @@ void build_kinetic(gpe_engine_s* e) {
     fill(e->Kz, e->kz, (double)e->n);
 }
 
-template <typename T>
+// kinetic factors stay FP64 in both precisions
 int upload_kinetic(gpe_engine_s* e) {
-    std::vector<cplx<T>> K;
-    for (auto& v : e->Kx) K.push_back(mk((T)v.real(), (T)v.imag()));
-    for (auto& v : e->Ky) K.push_back(mk((T)v.real(), (T)v.imag()));
-    for (auto& v : e->Kz) K.push_back(mk((T)v.real(), (T)v.imag()));
-    HIP_TRY(hipMemcpy(e->Ktab, K.data(), K.size() * sizeof(cplx<T>), hipMemcpyHostToDevice));
+    std::vector<cplx<double>> K;
+    for (auto& v : e->Kx) K.push_back(mk(v.real(), v.imag()));
+    for (auto& v : e->Ky) K.push_back(mk(v.real(), v.imag()));
+    for (auto& v : e->Kz) K.push_back(mk(v.real(), v.imag()));
+    HIP_TRY(hipMemcpy(e->Ktab, K.data(), K.size() * sizeof(cplx<double>), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -154,13 +156,16 @@ template <typename T>
 int upload_static_tables(gpe_engine_s* e) {
     const int dims[3] = {e->nx, e->ny, e->nz};
     for (int a = 0; a < 3; a++) {
-        std::vector<cplx<T>> tw(dims[a]);
+        std::vector<cplx<T>> tw(dims[a]), lo(dims[a]);
         for (int m = 0; m < dims[a]; m++) {
             double re, im;
             unit_root(m, dims[a], &re, &im);
             tw[m] = mk((T)re, (T)im);
+            lo[m] = mk((T)(re - (double)tw[m].x), (T)(im - (double)tw[m].y));
         }
         HIP_TRY(hipMemcpy(e->tw[a], tw.data(), tw.size() * sizeof(cplx<T>), hipMemcpyHostToDevice));
+        if (e->twlo[a])
+            HIP_TRY(hipMemcpy(e->twlo[a], lo.data(), lo.size() * sizeof(cplx<T>), hipMemcpyHostToDevice));
     }
     std::vector<T> k;
     std::vector<double> k64;
@@ -171,12 +176,10 @@ int upload_static_tables(gpe_engine_s* e) {
         }
     HIP_TRY(hipMemcpy(e->ktab, k.data(), k.size() * sizeof(T), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(e->k64, k64.data(), k64.size() * sizeof(double), hipMemcpyHostToDevice));
-    return upload_kinetic<T>(e);
+    return upload_kinetic(e);
 }
 
-int upload_kinetic_any(gpe_engine_s* e) {
-    return e->precision == GPE_PRECISION_C64 ? upload_kinetic<float>(e) : upload_kinetic<double>(e);
-}
+int upload_kinetic_any(gpe_engine_s* e) { return upload_kinetic(e); }
 
 int axis_len(const gpe_engine_s* e, int axis) { return axis == 0 ? e->nx : axis == 1 ? e->ny : e->nz; }
 
@@ -188,7 +191,7 @@ PassArgs<T> base_args(gpe_engine_s* e) {
     a.vbuf = (T*)e->Vb;
     a.vq = (T*)e->Vq;
     a.rout = e->rout;
-    const cplx<T>* K = (const cplx<T>*)e->Ktab;
+    const cplx<double>* K = (const cplx<double>*)e->Ktab;
     a.Kx = K;
     a.Ky = K + e->nx;
     a.Kz = K + e->nx + e->ny;
@@ -236,6 +239,7 @@ int run_pass(gpe_engine_s* e, PassArgs<T> a, const PassSpec& s) {
     a.src = (const cplx<T>*)s.src;
     a.dst = (cplx<T>*)s.dst;
     a.tw = (const cplx<T>*)e->tw[s.axis];
+    a.twlo = (const cplx<T>*)e->twlo[s.axis];
     a.dir1 = s.dir1;
     a.dir2 = s.dir2;
     a.load_mode = s.load;
@@ -424,7 +428,8 @@ int get_psi_t(gpe_engine_s* e, cuCplx* psi) {
 }
 
 void free_all(gpe_engine_s* e) {
-    void* ptrs[] = {e->A, e->B, e->C, e->Vb, e->Vq, e->varr, e->tw[0], e->tw[1], e->tw[2], e->Ktab, e->ktab,
+    void* ptrs[] = {e->A, e->B, e->C, e->Vb, e->Vq, e->varr, e->tw[0], e->tw[1], e->tw[2], e->twlo[0], e->twlo[1],
+                    e->twlo[2], e->Ktab, e->ktab,
                     e->k64, e->partials, e->scal, e->rout, e->phase, e->wind};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -484,8 +489,11 @@ int create_impl(gpe_engine_s** out, const gpe_config_t* cfg, double alpha, doubl
     ALLOC(e->A, n * 2 * es);
     ALLOC(e->B, n * 2 * es);
     ALLOC(e->Vb, n * es);
-    for (int a = 0; a < 3; a++) ALLOC(e->tw[a], (size_t)axis_len(e, a) * 2 * es);
-    ALLOC(e->Ktab, (size_t)(e->nx + e->ny + e->nz) * 2 * es);
+    for (int a = 0; a < 3; a++) {
+        ALLOC(e->tw[a], (size_t)axis_len(e, a) * 2 * es);
+        if (e->precision == GPE_PRECISION_C64) ALLOC(e->twlo[a], (size_t)axis_len(e, a) * 2 * es);
+    }
+    ALLOC(e->Ktab, (size_t)(e->nx + e->ny + e->nz) * sizeof(cplx<double>));
     ALLOC(e->ktab, (size_t)(e->nx + e->ny + e->nz) * es);
     ALLOC(e->k64, (size_t)(e->nx + e->ny + e->nz) * sizeof(double));
     // partial sums: the largest grid of any pass times three sums
@@ -901,6 +909,22 @@ int gpe_energy(double* t, double* ekin, double* eint, double* eext) {
 int gpe_set_external_potential(const double* v) { return gpe_engine_set_external_potential(g_eng, v); }
 int gpe_detect_vortices(double rho_min, int8_t* w, long* count) {
     return gpe_engine_detect_vortices(g_eng, rho_min, w, count);
+}
+
+int gpe_set_device(int device) {
+    HIP_TRY(hipSetDevice(device));
+    return 0;
+}
+
+int gpe_host_alloc(void** ptr, size_t bytes) {
+    if (!ptr) return GPE_ERR_ARG;
+    HIP_TRY(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+    return 0;
+}
+
+int gpe_host_free(void* ptr) {
+    if (ptr) HIP_TRY(hipHostFree(ptr));
+    return 0;
 }
 
 void gpe_print_potential_type(void) {

@@ -119,10 +119,25 @@ __device__ __forceinline__ cplx<T> rot(cplx<T> a) {
     } else {
         constexpr double cv = Roots16::c[j];
         constexpr double sv = S * Roots16::s[j];
-        const T c = (T)cv;
-        const T s = (T)sv;
-        return mk(a.x * c - a.y * s, a.x * s + a.y * c);
+        if constexpr (sizeof(T) == 4) {
+            // complex64: the constant as a float hi + lo pair, so that its
+            // rounding does not shrink |a| the same way on every application
+            constexpr float ch = (float)cv, cl = (float)(cv - (double)ch);
+            constexpr float sh = (float)sv, sl = (float)(sv - (double)sh);
+            return mk(fmaf(a.x, ch, fmaf(-a.y, sh, a.x * cl - a.y * sl)),
+                      fmaf(a.x, sh, fmaf(a.y, ch, a.x * sl + a.y * cl)));
+        } else {
+            const T c = (T)cv;
+            const T s = (T)sv;
+            return mk(a.x * c - a.y * s, a.x * s + a.y * c);
+        }
     }
+}
+
+// a * (hi + lo) for a complex64 twiddle stored as a hi + lo pair
+__device__ __forceinline__ cplx<float> cmul_comp(cplx<float> a, cplx<float> hi, cplx<float> lo) {
+    return mk(fmaf(a.x, hi.x, fmaf(-a.y, hi.y, a.x * lo.x - a.y * lo.y)),
+              fmaf(a.x, hi.y, fmaf(a.y, hi.x, a.x * lo.y + a.y * lo.x)));
 }
 
 template <int R, int S, typename T>
@@ -137,10 +152,17 @@ __device__ __forceinline__ void dft2(cplx<T>* v) {
 
 template <int S, typename T>
 __device__ __forceinline__ void dft3(cplx<T>* v) {
-    const T h = (T)0.86602540378443864676;  // sqrt(3)/2
+    constexpr double hd = 0.86602540378443864676;  // sqrt(3)/2
     cplx<T> t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
     cplx<T> m = mk(v[0].x - (T)0.5 * t1.x, v[0].y - (T)0.5 * t1.y);
-    cplx<T> it2 = mk(-(T)S * h * t2.y, (T)S * h * t2.x);
+    cplx<T> ht2;
+    if constexpr (sizeof(T) == 4) {
+        constexpr float hh = (float)hd, hl = (float)(hd - (double)hh);
+        ht2 = mk(fmaf(t2.x, hh, t2.x * hl), fmaf(t2.y, hh, t2.y * hl));
+    } else {
+        ht2 = mk((T)hd * t2.x, (T)hd * t2.y);
+    }
+    cplx<T> it2 = mk(-(T)S * ht2.y, (T)S * ht2.x);
     v[0] = cadd(v[0], t1);
     v[1] = cadd(m, it2);
     v[2] = csub(m, it2);
@@ -205,7 +227,7 @@ __device__ __forceinline__ void dft(cplx<T>* v) {
 // (w, j + r N/R), applies w_N^(r (j mod NS) N/(NS R)), transforms, and writes
 // (w, (j/NS) NS R + j mod NS + r NS).
 template <typename T, int N, int W, bool CONTIG, int S, int R, int NS>
-__device__ __forceinline__ void fft_stage(cplx<T>* lds, const cplx<T>* tw) {
+__device__ __forceinline__ void fft_stage(cplx<T>* lds, const cplx<T>* tw, const cplx<T>* twlo) {
     constexpr int NB = W * N / R;
     constexpr int PER = (NB + kThreads - 1) / kThreads;
     constexpr int M = N / R;
@@ -237,7 +259,13 @@ __device__ __forceinline__ void fft_stage(cplx<T>* lds, const cplx<T>* tw) {
                 for (int r = 1; r < R; r++) {
                     cplx<T> t = tw[k * r * TSTEP];
                     if (S > 0) t.y = -t.y;
-                    v[q][r] = cmul(v[q][r], t);
+                    if constexpr (sizeof(T) == 4) {
+                        cplx<T> tl = twlo[k * r * TSTEP];
+                        if (S > 0) tl.y = -tl.y;
+                        v[q][r] = cmul_comp(v[q][r], t, tl);
+                    } else {
+                        v[q][r] = cmul(v[q][r], t);
+                    }
                 }
             }
             dft<R, S>(v[q]);
@@ -250,11 +278,11 @@ __device__ __forceinline__ void fft_stage(cplx<T>* lds, const cplx<T>* tw) {
 }
 
 template <typename T, int N, int W, bool CONTIG, int S, int STAGE>
-__device__ __forceinline__ void fft_stages(cplx<T>* lds, const cplx<T>* tw) {
+__device__ __forceinline__ void fft_stages(cplx<T>* lds, const cplx<T>* tw, const cplx<T>* twlo) {
     constexpr FftPlan P = make_plan(N);
     if constexpr (STAGE < P.ns) {
-        fft_stage<T, N, W, CONTIG, S, P.r[STAGE], plan_span(P, STAGE)>(lds, tw);
-        fft_stages<T, N, W, CONTIG, S, STAGE + 1>(lds, tw);
+        fft_stage<T, N, W, CONTIG, S, P.r[STAGE], plan_span(P, STAGE)>(lds, tw, twlo);
+        fft_stages<T, N, W, CONTIG, S, STAGE + 1>(lds, tw, twlo);
     }
 }
 
@@ -297,13 +325,14 @@ struct PassArgs {
     const cplx<T>* src;
     cplx<T>* dst;
     const cplx<T>* tw;     // exp(-2 pi i m / N), m < N
+    const cplx<T>* twlo;   // complex64: exp(-2 pi i m / N) - tw, the low part
     cplx<T>* psiA;         // the wave function buffer (psi0 / psi1)
     T* vbuf;               // corrector potential V1 per point
     T* vq;                 // quantum-friction potential per point
     double* rout;          // real FP64 output (currents)
-    const cplx<T>* Kx;     // kinetic factors, Kz includes 1/N
-    const cplx<T>* Ky;
-    const cplx<T>* Kz;
+    const cplx<double>* Kx;  // kinetic factors in FP64, Kz includes 1/N
+    const cplx<double>* Ky;
+    const cplx<double>* Kz;
     const T* kx;           // wave numbers
     const T* ky;
     const T* kz;
@@ -345,6 +374,22 @@ __device__ __forceinline__ Coord coord(const PassArgs<T>& a, int N, unsigned p, 
     return c;
 }
 
+// v * exp(re + i im).  In complex64 the factor's angle and modulus come from
+// single-precision sincos/exp, the direction is renormalised to exact unit
+// modulus and the product taken in FP64, so the rounding of the factor does
+// not bias |psi| the same way every step.
+__device__ __forceinline__ cplx<double> mul_exp(cplx<double> v, double re, double im) {
+    return cmul(v, cexp_(re, im));
+}
+__device__ __forceinline__ cplx<float> mul_exp(cplx<float> v, float re, float im) {
+    float s, c;
+    sincosf(im, &s, &c);
+    const double cd = (double)c, sd = (double)s;
+    const double f = (double)expf(re) / sqrt(cd * cd + sd * sd);
+    const double vx = (double)v.x, vy = (double)v.y;
+    return mk((float)(f * (vx * cd - vy * sd)), (float)(f * (vx * sd + vy * cd)));
+}
+
 template <typename T>
 __device__ __forceinline__ T pot_at(const PassArgs<T>& a, const Coord& c, cplx<T> v, bool qf) {
     T lv = vext(a.ph, c.ix, c.iy, c.iz, c.off) + dedfdn(a.ph, density(a.ph, v));
@@ -357,7 +402,7 @@ template <typename T>
 __device__ __forceinline__ cplx<T> vstep1(const PassArgs<T>& a, const Coord& c, cplx<T> v, bool qf, T qval) {
     T lv = vext(a.ph, c.ix, c.iy, c.iz, c.off) + dedfdn(a.ph, density(a.ph, v));
     if (qf) lv = lv + qval;
-    return cmul(v, cexp_(a.ph.c.x * lv, a.ph.c.y * lv));
+    return mul_exp(v, a.ph.c.x * lv, a.ph.c.y * lv);
 }
 
 __device__ __forceinline__ double dens64(double g, double x, double y) { return g * (x * x + y * y); }
@@ -396,18 +441,22 @@ __device__ __forceinline__ cplx<T> load_op(const PassArgs<T>& a, const Coord& c,
         lv = (T)0.5 * (V0 + vext(a.ph, c.ix, c.iy, c.iz, c.off) + dedfdn(a.ph, density(a.ph, phi3)));
     a.vbuf[c.off] = lv;
     if (a.flags & F_IMPRINT_MID) psi0 = imprint(a.ph, psi0, c.ix, c.iy);
-    return cmul(psi0, cexp_(a.ph.c.x * lv, a.ph.c.y * lv));
+    return mul_exp(psi0, a.ph.c.x * lv, a.ph.c.y * lv);
 }
 
 template <typename T>
 __device__ __forceinline__ cplx<T> mid_op(const PassArgs<T>& a, const Coord& c, cplx<T> v, double* acc) {
     const bool qf = (a.flags & F_QF) != 0;
     switch (a.mid_mode) {
-    case MID_EXPT:
-        v = cmul(v, a.Kx[c.ix]);
-        v = cmul(v, a.Ky[c.iy]);
-        v = cmul(v, a.Kz[c.iz]);
-        return v;
+    case MID_EXPT: {
+        // orginal/gpe_engine.cu:880-895, in FP64 in both precisions: the factors
+        // are the same every step, so a rounded |K| != 1 would compound
+        cplx<double> w = mk((double)v.x, (double)v.y);
+        w = cmul(w, a.Kx[c.ix]);
+        w = cmul(w, a.Ky[c.iy]);
+        w = cmul(w, a.Kz[c.iz]);
+        return mk((T)w.x, (T)w.y);
+    }
     case MID_K2: {
         const T k2 = a.kx[c.ix] * a.kx[c.ix] + a.ky[c.iy] * a.ky[c.iy] + a.kz[c.iz] * a.kz[c.iz];
         return cscale(v, a.kfac * k2);
@@ -420,27 +469,27 @@ __device__ __forceinline__ cplx<T> mid_op(const PassArgs<T>& a, const Coord& c, 
         // orginal/gpe_engine.cu:774-801 (and the _part1/_part2[_qf] pair for qf)
         const cplx<T> psi0 = a.psiA[c.off];
         const T V0 = pot_at(a, c, psi0, qf);
-        const cplx<T> phi3 = cmul(v, cexp_(a.ph.c.x * V0, a.ph.c.y * V0));
+        const cplx<T> phi3 = mul_exp(v, a.ph.c.x * V0, a.ph.c.y * V0);
         T lv;
         if (qf)
             lv = (T)0.5 * (V0 + vext(a.ph, c.ix, c.iy, c.iz, c.off) + dedfdn(a.ph, density(a.ph, phi3)) + a.vq[c.off]);
         else
             lv = (T)0.5 * (V0 + vext(a.ph, c.ix, c.iy, c.iz, c.off) + dedfdn(a.ph, density(a.ph, phi3)));
         a.vbuf[c.off] = lv;
-        return cmul(psi0, cexp_(a.ph.c.x * lv, a.ph.c.y * lv));
+        return mul_exp(psi0, a.ph.c.x * lv, a.ph.c.y * lv);
     }
     case MID_PART1: {
         // orginal/gpe_engine.cu:803-815 with the density sum of gpe_normalize fused
         const cplx<T> psi0 = a.psiA[c.off];
         const T V0 = pot_at(a, c, psi0, qf);
-        const cplx<T> phi3 = cmul(v, cexp_(a.ph.c.x * V0, a.ph.c.y * V0));
+        const cplx<T> phi3 = mul_exp(v, a.ph.c.x * V0, a.ph.c.y * V0);
         acc[0] += dens64((double)a.ph.gamma, (double)phi3.x, (double)phi3.y);
         return phi3;
     }
     case MID_VSTEP3: {
         // orginal/gpe_engine.cu:870-878; the factor exp(c V1) is recomputed from V1
         const T lv = a.vbuf[c.off];
-        cplx<T> psi1 = cmul(v, cexp_(a.ph.c.x * lv, a.ph.c.y * lv));
+        cplx<T> psi1 = mul_exp(v, a.ph.c.x * lv, a.ph.c.y * lv);
         if (a.flags & F_RED_NORM) acc[0] += dens64((double)a.ph.gamma, (double)psi1.x, (double)psi1.y);
         if (a.flags & (F_NEXT_VSTEP1 | F_NEXT_PSI)) {
             if (a.flags & F_IMPRINT_NEXT) psi1 = imprint(a.ph, psi1, c.ix, c.iy);
@@ -495,9 +544,13 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(const PassArgs<T> a)
     constexpr int PERE = (TILE + kThreads - 1) / kThreads;
     __shared__ cplx<T> lds[lds_elems<N, W, CONTIG>()];
     __shared__ cplx<T> tw[N];
+    __shared__ cplx<T> twlo[sizeof(T) == 4 ? N : 1];
     __shared__ double red[kThreads / 64];
 
-    for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = a.tw[i];
+    for (int i = threadIdx.x; i < N; i += kThreads) {
+        tw[i] = a.tw[i];
+        if constexpr (sizeof(T) == 4) twlo[i] = a.twlo[i];
+    }
     const unsigned p0 = blockIdx.x * (unsigned)W;
     double acc[3] = {0.0, 0.0, 0.0};
 
@@ -514,8 +567,8 @@ for (int q = 0; q < PERE; q++) {
         }
     }
     __syncthreads();
-    if (a.dir1 < 0) fft_stages<T, N, W, CONTIG, -1, 0>(lds, tw);
-    else fft_stages<T, N, W, CONTIG, 1, 0>(lds, tw);
+    if (a.dir1 < 0) fft_stages<T, N, W, CONTIG, -1, 0>(lds, tw, twlo);
+    else fft_stages<T, N, W, CONTIG, 1, 0>(lds, tw, twlo);
 
     if (a.mid_mode != MID_NONE) {
 for (int q = 0; q < PERE; q++) {
@@ -533,8 +586,8 @@ for (int q = 0; q < PERE; q++) {
         }
         __syncthreads();
     }
-    if (a.dir2 < 0) fft_stages<T, N, W, CONTIG, -1, 0>(lds, tw);
-    else if (a.dir2 > 0) fft_stages<T, N, W, CONTIG, 1, 0>(lds, tw);
+    if (a.dir2 < 0) fft_stages<T, N, W, CONTIG, -1, 0>(lds, tw, twlo);
+    else if (a.dir2 > 0) fft_stages<T, N, W, CONTIG, 1, 0>(lds, tw, twlo);
 
     if (a.store_mode == STORE_PLAIN) {
 for (int q = 0; q < PERE; q++) {

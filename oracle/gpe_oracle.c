@@ -47,6 +47,7 @@ struct gpeo {
     double *varr;         /* VEXT 3 array */
     double vx0, vy0;
     int vQ;
+    double *phase;        /* enforced phase (d_phase) */
 };
 
 /* ------------------------------------------------------------------ hooks */
@@ -267,7 +268,7 @@ void gpeo_destroy(gpeo_t *e)
     free(e->kx); free(e->ky); free(e->kz);
     free(e->Kx); free(e->Ky); free(e->Kz);
     free(e->psi); free(e->psi2); free(e->wrkR); free(e->wrk);
-    free(e->qfpot); free(e->varr);
+    free(e->qfpot); free(e->varr); free(e->phase);
     free(e);
 }
 
@@ -472,11 +473,63 @@ static void imprint_vortex(gpeo_t *e, cd *psi)
         }
 }
 
-/* One split step shared by gpe_evolve_qf (orginal/gpe_engine.cu:948-1013,
- * dev/gpe_engine.cu:1035-1106) and gpe_evolve_vortex (dev/gpe_engine.cu:1120-1199). */
-static void one_step(gpeo_t *e, int vortex, double *mu)
+/* __gpe_imprint2_vortexline_zdir_: dev/gpe_engine.cu:520-548, the imprint
+ * only where |psi| > 1e-15 */
+static void imprint_vortex2(gpeo_t *e, cd *psi)
 {
-    if (vortex) imprint_vortex(e, e->psi);
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++)
+        for (int iy = 0; iy < e->ny; iy++) {
+            double x = (double)ix - 1.0 * (e->nx / 2) - e->vx0;
+            double y = (double)iy - 1.0 * (e->ny / 2) - e->vy0;
+            double ph = atan2(x, y) * (double)e->vQ;
+            double cp = cos(ph), sp = sin(ph);
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                double a = hypot(psi[i].re, psi[i].im);
+                if (a > 1e-15) {
+                    psi[i].re = a * cp;
+                    psi[i].im = a * sp;
+                }
+            }
+        }
+}
+
+/* __gpe_enforce_phase__: dev/gpe_engine.cu:584-591, psi = |psi| exp(i phase) */
+static void enforce_phase(gpeo_t *e, cd *psi)
+{
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < e->n; i++) {
+        double a = sqrt(psi[i].re * psi[i].re + psi[i].im * psi[i].im);
+        psi[i].re = a * cos(e->phase[i]);
+        psi[i].im = a * sin(e->phase[i]);
+    }
+}
+
+enum { PIN_NONE = 0, PIN_VORTEX = 1, PIN_VORTEX2 = 2, PIN_VORTEX3 = 3, PIN_PHASE = 4 };
+
+static void pin_start(gpeo_t *e, int pin)
+{
+    if (pin == PIN_VORTEX || pin == PIN_VORTEX3) imprint_vortex(e, e->psi);
+    else if (pin == PIN_VORTEX2) imprint_vortex2(e, e->psi);
+    else if (pin == PIN_PHASE) enforce_phase(e, e->psi);
+}
+
+/* re-applied after the predictor normalisation in imaginary time, except by
+ * gpe_evolve_vortex3 (dev/gpe_engine.cu:1160-1162, 1252-1254, 1344-1346, 1436-1438) */
+static void pin_mid(gpeo_t *e, int pin)
+{
+    if (pin == PIN_VORTEX) imprint_vortex(e, e->psi);
+    else if (pin == PIN_VORTEX2) imprint_vortex2(e, e->psi);
+    else if (pin == PIN_PHASE) enforce_phase(e, e->psi);
+}
+
+/* One split step shared by gpe_evolve_qf (orginal/gpe_engine.cu:948-1013,
+ * dev/gpe_engine.cu:1035-1106), gpe_evolve_vortex / 2 / 3 (dev/gpe_engine.cu:
+ * 1120-1199, 1207-1294, 1299-1386) and gpe_evolve_enforced_phase (1390-1477). */
+static void one_step(gpeo_t *e, int pin, double *mu)
+{
+    pin_start(e, pin);
 
     if (e->qf != 0.0) {
         compute_qf_potential(e, e->psi, e->wrk, e->qfpot);
@@ -495,7 +548,7 @@ static void one_step(gpeo_t *e, int vortex, double *mu)
         vstep2_part1(e, e->psi2, e->wrkR);
         if (e->beta != 0.0) {
             normalize(e, e->psi2);
-            if (vortex) imprint_vortex(e, e->psi);
+            pin_mid(e, pin);
         }
         if (e->qf != 0.0) {
             compute_qf_potential(e, e->psi, e->wrk, e->qfpot);
@@ -520,13 +573,49 @@ static void one_step(gpeo_t *e, int vortex, double *mu)
 
 int gpeo_evolve_qf(gpeo_t *e, int nt, double *mu)
 {
-    for (int i = 0; i < nt; i++) one_step(e, 0, mu);
+    for (int i = 0; i < nt; i++) one_step(e, PIN_NONE, mu);
     return 0;
 }
 
 int gpeo_evolve_vortex(gpeo_t *e, int nt, double *mu)
 {
-    for (int i = 0; i < nt; i++) one_step(e, 1, mu);
+    for (int i = 0; i < nt; i++) one_step(e, PIN_VORTEX, mu);
+    return 0;
+}
+
+int gpeo_evolve_vortex2(gpeo_t *e, int nt, double *mu)
+{
+    for (int i = 0; i < nt; i++) one_step(e, PIN_VORTEX2, mu);
+    return 0;
+}
+
+int gpeo_evolve_vortex3(gpeo_t *e, int nt, double *mu)
+{
+    for (int i = 0; i < nt; i++) one_step(e, PIN_VORTEX3, mu);
+    return 0;
+}
+
+int gpeo_evolve_enforced_phase(gpeo_t *e, int nt, double *mu)
+{
+    if (!e->phase) return -15;
+    for (int i = 0; i < nt; i++) one_step(e, PIN_PHASE, mu);
+    return 0;
+}
+
+/* dev/gpe_engine.cu:561-570 */
+int gpeo_set_phase(gpeo_t *e, const double *phase)
+{
+    if (!e->phase) e->phase = (double *)malloc(sizeof(double) * e->n);
+    memcpy(e->phase, phase, sizeof(double) * e->n);
+    return 0;
+}
+
+/* dev/gpe_engine.cu:552-559, 572-582: phase = arg psi, kept as the enforced phase */
+int gpeo_get_phase(gpeo_t *e, double *phase)
+{
+    if (!e->phase) e->phase = (double *)malloc(sizeof(double) * e->n);
+    for (size_t i = 0; i < e->n; i++) e->phase[i] = atan2(e->psi[i].im, e->psi[i].re);
+    if (phase) memcpy(phase, e->phase, sizeof(double) * e->n);
     return 0;
 }
 

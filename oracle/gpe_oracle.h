@@ -54,6 +54,11 @@ int gpeo_get_norm(gpeo_t *e, double *norm);
 
 int gpeo_evolve_qf(gpeo_t *e, int nt, double *mu);
 int gpeo_evolve_vortex(gpeo_t *e, int nt, double *mu);
+int gpeo_evolve_vortex2(gpeo_t *e, int nt, double *mu);
+int gpeo_evolve_vortex3(gpeo_t *e, int nt, double *mu);
+int gpeo_evolve_enforced_phase(gpeo_t *e, int nt, double *mu);
+int gpeo_set_phase(gpeo_t *e, const double *phase);
+int gpeo_get_phase(gpeo_t *e, double *phase);
 int gpeo_energy(gpeo_t *e, double *t, double *ekin, double *eint, double *eext);
 int gpeo_get_density(gpeo_t *e, double *t, double *rho);
 int gpeo_get_currents(gpeo_t *e, double *t, double *jx, double *jy, double *jz);

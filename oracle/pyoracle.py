@@ -53,6 +53,11 @@ def lib():
             ("gpeo_get_norm", [_p, _dp]),
             ("gpeo_evolve_qf", [_p, _i, _dp]),
             ("gpeo_evolve_vortex", [_p, _i, _dp]),
+            ("gpeo_evolve_vortex2", [_p, _i, _dp]),
+            ("gpeo_evolve_vortex3", [_p, _i, _dp]),
+            ("gpeo_evolve_enforced_phase", [_p, _i, _dp]),
+            ("gpeo_set_phase", [_p, _p]),
+            ("gpeo_get_phase", [_p, _p]),
             ("gpeo_energy", [_p, _dp, _dp, _dp, _dp]),
             ("gpeo_get_density", [_p, _dp, _p]),
             ("gpeo_get_currents", [_p, _dp, _p, _p, _p]),
@@ -189,6 +194,31 @@ class OracleEngine:
         mu = ctypes.c_double(float("nan"))
         assert lib().gpeo_evolve_vortex(self._h, nt, ctypes.byref(mu) if want_mu else None) == 0
         return mu.value if want_mu else None
+
+    def _evolve_variant(self, fn, nt, want_mu):
+        mu = ctypes.c_double(float("nan"))
+        rc = fn(self._h, nt, ctypes.byref(mu) if want_mu else None)
+        if rc != 0:
+            raise RuntimeError("oracle evolve failed: %d" % rc)
+        return mu.value if want_mu else None
+
+    def evolve_vortex2(self, nt, want_mu=False):
+        return self._evolve_variant(lib().gpeo_evolve_vortex2, nt, want_mu)
+
+    def evolve_vortex3(self, nt, want_mu=False):
+        return self._evolve_variant(lib().gpeo_evolve_vortex3, nt, want_mu)
+
+    def evolve_enforced_phase(self, nt, want_mu=False):
+        return self._evolve_variant(lib().gpeo_evolve_enforced_phase, nt, want_mu)
+
+    def set_phase(self, phase):
+        a = np.ascontiguousarray(phase, dtype=np.float64).reshape(self.shape)
+        assert lib().gpeo_set_phase(self._h, _ptr(a)) == 0
+
+    def get_phase(self):
+        a = np.empty(self.shape, dtype=np.float64)
+        assert lib().gpeo_get_phase(self._h, _ptr(a)) == 0
+        return a
 
     def energy(self):
         t, ek, ei, ee = (ctypes.c_double() for _ in range(4))
