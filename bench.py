@@ -137,11 +137,17 @@ def main():
     t_prof = timed(eng, a.steps)
     ms_strided, n_strided = eng.profile_read()
     eng.profile(0, False)
-    eng.profile(1, True)
-    eng.evolve_qf(a.steps)
-    ms_z, n_z = eng.profile_read()
-    eng.profile(1, False)
-    total_steps = a.warmup + 3 * a.steps
+    # per-axis split of the same kernels (diagnostic), 1/10 of the steps each
+    sub = max(a.steps // 10, 1)
+    per_axis = {}
+    for cls, name in ((1, "z"), (2, "y"), (3, "x")):
+        eng.profile(cls, True)
+        eng.evolve_qf(sub)
+        ms, cnt = eng.profile_read()
+        eng.profile(cls, False)
+        per_axis[name] = ms / max(cnt, 1)
+    copy_ms = min(eng.copy_benchmark() for _ in range(5))
+    total_steps = a.warmup + 2 * a.steps + 3 * sub
     _, ek, ei, ee = eng.energy()
     e64 = ek + ei + ee
     eng.close()
@@ -162,7 +168,6 @@ def main():
         "launches_timed": n_strided,
         "algorithmic_bytes_per_launch": bytes_launch,
     }
-    z_avg = ms_z / max(n_z, 1)
 
     result = {
         "metric": METRIC,
@@ -185,7 +190,8 @@ def main():
         },
         "step_hbm_frac": 22 * S * sps / (HBM_PEAK_GBS * 1e9),
         "step_algorithmic_bytes": 22 * S,
-        "z_pass_avg_ms": z_avg,
+        "pass_avg_ms": per_axis,
+        "copy_kernel_GBs": 2 * S / (copy_ms * 1e-3) / 1e9,
         "profiled_region_steps_per_s": a.steps / t_prof,
         "roofline": roofline,
     }

@@ -111,6 +111,12 @@ int gpe_evolve(int nt);
 int gpe_evolve_qf(int nt, double *chemical_potential GPE_DEFAULT(NULL));
 /* dev/gpe_engine.h:465 */
 int gpe_evolve_vortex(int nt, double *chemical_potential GPE_DEFAULT(NULL));
+/* dev/gpe_engine.h:475: imprint only where |psi| > 1e-15 */
+int gpe_evolve_vortex2(int nt, double *chemical_potential GPE_DEFAULT(NULL));
+/* dev/gpe_engine.h:485: imprint in the predictor only */
+int gpe_evolve_vortex3(int nt, double *chemical_potential GPE_DEFAULT(NULL));
+/* dev/gpe_engine.h:494: phase from gpe_set_phase / gpe_get_phase enforced */
+int gpe_evolve_enforced_phase(int nt, double *chemical_potential GPE_DEFAULT(NULL));
 /* dev/gpe_engine.h:545 */
 int gpe_energy(double *t, double *ekin, double *eint, double *eext);
 /* dev/gpe_engine.h:552 */
@@ -164,6 +170,9 @@ int gpe_engine_normalize_psi(gpe_engine_t e);
 int gpe_engine_get_norm(gpe_engine_t e, double *norm);
 int gpe_engine_evolve_qf(gpe_engine_t e, int nt, double *chemical_potential);
 int gpe_engine_evolve_vortex(gpe_engine_t e, int nt, double *chemical_potential);
+int gpe_engine_evolve_vortex2(gpe_engine_t e, int nt, double *chemical_potential);
+int gpe_engine_evolve_vortex3(gpe_engine_t e, int nt, double *chemical_potential);
+int gpe_engine_evolve_enforced_phase(gpe_engine_t e, int nt, double *chemical_potential);
 int gpe_engine_energy(gpe_engine_t e, double *t, double *ekin, double *eint, double *eext);
 int gpe_engine_get_density(gpe_engine_t e, double *t, double *density);
 int gpe_engine_get_currents(gpe_engine_t e, double *t, double *jx, double *jy, double *jz);
@@ -182,10 +191,14 @@ int gpe_engine_device_psi(gpe_engine_t e, void **ptr);
 int gpe_engine_stream(gpe_engine_t e, void **stream);
 int gpe_engine_synchronize(gpe_engine_t e);
 /* Kernel timing with HIP events on the engine's stream: while enabled, every
- * FFT pass launched on the selected axis class (0: strided y/x passes,
- * 1: contiguous z passes) is bracketed by an event pair; read returns the
- * summed milliseconds and the number of timed launches. */
+ * FFT pass launched on the selected axis class (0: strided y and x passes,
+ * 1: contiguous z passes, 2: y passes, 3: x passes) is bracketed by an event
+ * pair; read returns the summed milliseconds and the number of timed launches. */
 int gpe_engine_profile(gpe_engine_t e, int axis_class, int enable);
+/* one plain copy kernel of the state array (psi -> work buffer), timed with
+ * HIP events: the streaming-bandwidth reference of the bench and the
+ * calibration point of hardware byte counters */
+int gpe_engine_copy_benchmark(gpe_engine_t e, double *ms);
 int gpe_engine_profile_read(gpe_engine_t e, double *total_ms, long *launches);
 /* library build information (GPU target, compiled FFT lengths) */
 const char *gpe_build_info(void);

@@ -90,6 +90,7 @@ struct Phys {
     cplx<R> c;               // step coefficient gamma*dt/2/(-beta + i alpha)
     R up[kMaxUserParams];    // user parameters (d_user_param)
     const R* varr;           // VEXT_ARRAY potential
+    const R* vtab;           // separable potentials: per-axis terms [nx | ny | nz]
     R vx0, vy0;              // vortex line position
     R vQ;                    // vortex charge
 };
@@ -104,24 +105,19 @@ __device__ __forceinline__ R density(const Phys<R>& p, cplx<R> v) {
 template <typename R>
 __device__ __forceinline__ R vext(const Phys<R>& p, unsigned ix, unsigned iy, unsigned iz, unsigned off) {
     switch (p.vext) {
-    case VEXT_HARMONIC: {
-        const R x = (R)ix - (R)1.0 * (R)(p.nx / 2);
-        const R y = (R)iy - (R)1.0 * (R)(p.ny / 2);
-        const R z = (R)iz - (R)1.0 * (R)(p.nz / 2);
-        return (R)0.5 * x * x * p.up[0] * p.up[0] + (R)0.5 * y * y * p.up[1] * p.up[1] +
-               (R)0.5 * z * z * p.up[2] * p.up[2];
-    }
+    case VEXT_HARMONIC:
+        // 0.5 x^2 wx^2 + 0.5 y^2 wy^2 + 0.5 z^2 wz^2, each term tabulated per axis
+        return p.vtab[ix] + p.vtab[p.nx + iy] + p.vtab[p.nx + p.ny + iz];
     case VEXT_WOODS_SAXON: {
         const R x = (R)ix - (R)1.0 * (R)(p.nx / 2);
         const R y = (R)iy - (R)1.0 * (R)(p.ny / 2);
         const R r = g_sqrt(x * x + y * y);
         return p.up[0] * ((R)1. - (R)1. / ((R)1 + g_exp((r - p.up[2]) / p.up[1])));
     }
-    case VEXT_COSINE: {
-        // A (cos(2 pi P ix/nx) + cos(2 pi P iy/ny) + cos(2 pi P iz/nz)), A = up[0], P = up[1]
-        const R tp = (R)(2.0 * 3.14159265358979323846) * p.up[1];
-        return p.up[0] * (g_cos(tp * (R)ix / (R)p.nx) + g_cos(tp * (R)iy / (R)p.ny) + g_cos(tp * (R)iz / (R)p.nz));
-    }
+    case VEXT_COSINE:
+        // A (cos(2 pi P ix/nx) + cos(2 pi P iy/ny) + cos(2 pi P iz/nz)), A = up[0], P = up[1],
+        // the cosines tabulated per axis
+        return p.up[0] * (p.vtab[ix] + p.vtab[p.nx + iy] + p.vtab[p.nx + p.ny + iz]);
     case VEXT_ARRAY:
         return p.varr[off];
     default:

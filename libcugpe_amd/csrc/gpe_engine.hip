@@ -67,6 +67,7 @@ struct gpe_engine_s {
     void *tw[3] = {nullptr, nullptr, nullptr};        // per-axis twiddles
     void *twlo[3] = {nullptr, nullptr, nullptr};      // complex64: low parts of the twiddles
     void *Ktab = nullptr, *ktab = nullptr;           // kinetic factors / wave numbers
+    void *vtab = nullptr;                            // per-axis terms of separable potentials
     double *k64 = nullptr;
     double *partials = nullptr;
     size_t partials_len = 0;
@@ -181,6 +182,35 @@ int upload_static_tables(gpe_engine_s* e) {
 
 int upload_kinetic_any(gpe_engine_s* e) { return upload_kinetic(e); }
 
+// Per-axis terms of the separable external potentials, so that kernels add
+// three table entries instead of evaluating the hook:
+//   harmonic  0.5 x^2 w_a^2, x = i - n/2        (orginal/gpe_user_defined.h:79-101)
+//   cosine    cos(2 pi P i / n)                  (multiplied by A in the kernel)
+template <typename T>
+int upload_potential_tables(gpe_engine_s* e) {
+    std::vector<T> v;
+    const int dims[3] = {e->nx, e->ny, e->nz};
+    for (int a = 0; a < 3; a++)
+        for (int i = 0; i < dims[a]; i++) {
+            double t = 0.0;
+            if (e->vext == VEXT_HARMONIC) {
+                const double x = (double)i - 1.0 * (dims[a] / 2);
+                t = 0.5 * x * x * e->up[a] * e->up[a];
+            } else if (e->vext == VEXT_COSINE) {
+                const double tp = 2.0 * M_PI * e->up[1];
+                t = std::cos(tp * (double)i / (double)dims[a]);
+            }
+            v.push_back((T)t);
+        }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMemcpy(e->vtab, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int upload_potential_any(gpe_engine_s* e) {
+    return e->precision == GPE_PRECISION_C64 ? upload_potential_tables<float>(e) : upload_potential_tables<double>(e);
+}
+
 int axis_len(const gpe_engine_s* e, int axis) { return axis == 0 ? e->nx : axis == 1 ? e->ny : e->nz; }
 
 template <typename T>
@@ -216,6 +246,7 @@ PassArgs<T> base_args(gpe_engine_s* e) {
     p.c = mk((T)e->c.real(), (T)e->c.imag());
     for (int i = 0; i < kMaxUserParams; i++) p.up[i] = (T)e->up[i];
     p.varr = (const T*)e->varr;
+    p.vtab = (const T*)e->vtab;
     p.vx0 = (T)e->vx0;
     p.vy0 = (T)e->vy0;
     p.vQ = (T)e->vQ;
@@ -236,6 +267,7 @@ int run_pass(gpe_engine_s* e, PassArgs<T> a, const PassSpec& s) {
     a.axis = s.axis;
     const int N = axis_len(e, s.axis);
     a.M = (unsigned)(e->n / (size_t)N);
+    a.n1 = (unsigned)e->n;
     a.src = (const cplx<T>*)s.src;
     a.dst = (cplx<T>*)s.dst;
     a.tw = (const cplx<T>*)e->tw[s.axis];
@@ -247,8 +279,10 @@ int run_pass(gpe_engine_s* e, PassArgs<T> a, const PassSpec& s) {
     a.store_mode = s.store;
     a.flags = s.flags;
     a.nred = s.nred;
-    const int cls = (s.axis == 2) ? 1 : 0;
-    const bool prof = (e->prof_class == cls) && (e->prof_used < e->prof_ev.size());
+    // profiling classes: 0 strided (x and y) passes, 1 z passes, 2 y passes, 3 x passes
+    const bool match = (e->prof_class == 0 && s.axis != 2) || (e->prof_class == 1 && s.axis == 2) ||
+                       (e->prof_class == 2 && s.axis == 1) || (e->prof_class == 3 && s.axis == 0);
+    const bool prof = match && (e->prof_used < e->prof_ev.size());
     if (prof) (void)hipEventRecord(e->prof_ev[e->prof_used].a, e->stream);
     const unsigned blocks = launch_fft_pass<T>(N, a, e->stream);
     if (prof) {
@@ -273,11 +307,14 @@ int normalize_t(gpe_engine_s* e) {
 }
 
 // nt split steps (orginal/gpe_engine.cu:942-1016, dev/gpe_engine.cu:1030-1202)
+// pin: 0 none, 1 vortex imprint, 2 imprint where |psi| > 1e-15, 3 stored phase;
+// mid_pin: re-applied after the predictor normalisation in imaginary time
 template <typename T>
-int evolve_t(gpe_engine_s* e, int nt, double* mu, bool vortex) {
+int evolve_t(gpe_engine_s* e, int nt, double* mu, int pin, bool mid_pin) {
     if (nt <= 0) return 0;
     const bool ite = (e->beta != 0.0);
     const bool qf = (e->qf != 0.0);
+    const bool vortex = (pin != 0);
     const int imp = vortex ? F_IMPRINT : 0;
     const int fqf = qf ? F_QF : 0;
     void *A = e->A, *B = e->B, *C = e->C;
@@ -285,6 +322,8 @@ int evolve_t(gpe_engine_s* e, int nt, double* mu, bool vortex) {
 
     PassArgs<T> a = base_args<T>(e);
     a.kfac = (T)e->qf;
+    a.pin = pin;
+    a.phase = e->phase;
     if (!qf)
         run_pass<T>(e, a, {2, nullptr, B, F, 0, LOAD_PSI0, MID_NONE, STORE_PLAIN, F_VSTEP1 | imp, 0});
     else
@@ -309,7 +348,7 @@ int evolve_t(gpe_engine_s* e, int nt, double* mu, bool vortex) {
             const int nb = run_pass<T>(e, a, {2, B, B, I, 0, LOAD_PLAIN, MID_PART1, STORE_PLAIN, fqf, 1});
             finalize(e, nb, 1, FIN_NORM, 0, 1.0);
             run_pass<T>(e, a, {2, B, A, F, 0, LOAD_PART2, MID_NONE, STORE_PLAIN,
-                               F_SCALE | fqf | (vortex ? F_IMPRINT_MID : 0), 0});
+                               F_SCALE | fqf | (mid_pin ? F_IMPRINT_MID : 0), 0});
         }
         // corrector kinetic step of modify(psi0) W
         run_pass<T>(e, a, {1, A, A, F, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
@@ -429,7 +468,7 @@ int get_psi_t(gpe_engine_s* e, cuCplx* psi) {
 
 void free_all(gpe_engine_s* e) {
     void* ptrs[] = {e->A, e->B, e->C, e->Vb, e->Vq, e->varr, e->tw[0], e->tw[1], e->tw[2], e->twlo[0], e->twlo[1],
-                    e->twlo[2], e->Ktab, e->ktab,
+                    e->twlo[2], e->Ktab, e->ktab, e->vtab,
                     e->k64, e->partials, e->scal, e->rout, e->phase, e->wind};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -495,6 +534,7 @@ int create_impl(gpe_engine_s** out, const gpe_config_t* cfg, double alpha, doubl
     }
     ALLOC(e->Ktab, (size_t)(e->nx + e->ny + e->nz) * sizeof(cplx<double>));
     ALLOC(e->ktab, (size_t)(e->nx + e->ny + e->nz) * es);
+    ALLOC(e->vtab, (size_t)(e->nx + e->ny + e->nz) * es);
     ALLOC(e->k64, (size_t)(e->nx + e->ny + e->nz) * sizeof(double));
     // partial sums: the largest grid of any pass times three sums
     size_t maxb = kMaxReduceBlocks;
@@ -510,6 +550,7 @@ int create_impl(gpe_engine_s** out, const gpe_config_t* cfg, double alpha, doubl
     (void)hipMemset(e->A, 0, n * 2 * es);
     (void)hipMemset(e->scal, 0, 16 * sizeof(double));
     int r = (e->precision == GPE_PRECISION_C64) ? upload_static_tables<float>(e) : upload_static_tables<double>(e);
+    if (!r) r = upload_potential_any(e);
     if (r) return fail(r);
     *out = e;
     return 0;
@@ -580,7 +621,7 @@ int gpe_engine_set_user_params(gpe_engine_t e, int size, const double* params) {
     if (size > MAX_USER_PARAMS || size > kMaxUserParams) return GPE_ERR_TOO_MANY_PARAMS;
     if (size < 0 || (size > 0 && !params)) return GPE_ERR_ARG;
     for (int i = 0; i < kMaxUserParams; i++) e->up[i] = (i < size) ? params[i] : 0.0;
-    return 0;
+    return upload_potential_any(e);
 }
 
 // orginal/gpe_engine.cu:637-662
@@ -664,18 +705,25 @@ int gpe_engine_get_norm(gpe_engine_t e, double* norm) {
     return 0;
 }
 
-int gpe_engine_evolve_qf(gpe_engine_t e, int nt, double* mu) {
+static int evolve_any(gpe_engine_t e, int nt, double* mu, int pin, bool mid_pin) {
     if (!e) return GPE_ERR_NO_ENGINE;
+    if ((pin == 1 || pin == 2) && !e->vortex_set) return GPE_ERR_VORTEX_NOT_SET;
+    if (pin == 3 && !e->phase_set) return GPE_ERR_PHASE_NOT_SET;
     if (e->vext == VEXT_ARRAY && !e->varr_set) return GPE_ERR_ARG;
-    return e->precision == GPE_PRECISION_C64 ? evolve_t<float>(e, nt, mu, false) : evolve_t<double>(e, nt, mu, false);
+    return e->precision == GPE_PRECISION_C64 ? evolve_t<float>(e, nt, mu, pin, mid_pin)
+                                             : evolve_t<double>(e, nt, mu, pin, mid_pin);
 }
 
-int gpe_engine_evolve_vortex(gpe_engine_t e, int nt, double* mu) {
-    if (!e) return GPE_ERR_NO_ENGINE;
-    if (!e->vortex_set) return GPE_ERR_VORTEX_NOT_SET;
-    if (e->vext == VEXT_ARRAY && !e->varr_set) return GPE_ERR_ARG;
-    return e->precision == GPE_PRECISION_C64 ? evolve_t<float>(e, nt, mu, true) : evolve_t<double>(e, nt, mu, true);
-}
+// orginal/gpe_engine.cu:942-1016, dev/gpe_engine.cu:1030-1109
+int gpe_engine_evolve_qf(gpe_engine_t e, int nt, double* mu) { return evolve_any(e, nt, mu, 0, false); }
+// dev/gpe_engine.cu:1115-1202: imprint at every step start and after the ITE predictor
+int gpe_engine_evolve_vortex(gpe_engine_t e, int nt, double* mu) { return evolve_any(e, nt, mu, 1, true); }
+// dev/gpe_engine.cu:1207-1294: the same with the imprint only where |psi| > 1e-15
+int gpe_engine_evolve_vortex2(gpe_engine_t e, int nt, double* mu) { return evolve_any(e, nt, mu, 2, true); }
+// dev/gpe_engine.cu:1299-1386: imprint at the step start only
+int gpe_engine_evolve_vortex3(gpe_engine_t e, int nt, double* mu) { return evolve_any(e, nt, mu, 1, false); }
+// dev/gpe_engine.cu:1390-1477: the stored phase enforced like the vortex imprint
+int gpe_engine_evolve_enforced_phase(gpe_engine_t e, int nt, double* mu) { return evolve_any(e, nt, mu, 3, true); }
 
 int gpe_engine_energy(gpe_engine_t e, double* t, double* ekin, double* eint, double* eext) {
     if (!e) return GPE_ERR_NO_ENGINE;
@@ -795,7 +843,7 @@ int gpe_engine_profile(gpe_engine_t e, int axis_class, int enable) {
         e->prof_class = -1;
         return 0;
     }
-    if (axis_class != 0 && axis_class != 1) return GPE_ERR_ARG;
+    if (axis_class < 0 || axis_class > 3) return GPE_ERR_ARG;
     if (e->prof_ev.empty()) {
         e->prof_ev.resize(8192);
         for (auto& ev : e->prof_ev) {
@@ -819,6 +867,24 @@ int gpe_engine_profile_read(gpe_engine_t e, double* total_ms, long* launches) {
     }
     *total_ms = tot;
     *launches = (long)e->prof_used;
+    return 0;
+}
+
+int gpe_engine_copy_benchmark(gpe_engine_t e, double* ms) {
+    if (!e) return GPE_ERR_NO_ENGINE;
+    const size_t bytes = e->n * 2 * esize(e);
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    HIP_TRY(hipEventRecord(a, e->stream));
+    launch_copy(e->B, e->A, bytes, e->stream);
+    HIP_TRY(hipEventRecord(b, e->stream));
+    HIP_TRY(hipEventSynchronize(b));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (ms) *ms = t;
     return 0;
 }
 
@@ -903,6 +969,9 @@ int gpe_get_norm(double* norm) { return gpe_engine_get_norm(g_eng, norm); }
 int gpe_evolve(int nt) { return gpe_engine_evolve_qf(g_eng, nt, nullptr); }
 int gpe_evolve_qf(int nt, double* mu) { return gpe_engine_evolve_qf(g_eng, nt, mu); }
 int gpe_evolve_vortex(int nt, double* mu) { return gpe_engine_evolve_vortex(g_eng, nt, mu); }
+int gpe_evolve_vortex2(int nt, double* mu) { return gpe_engine_evolve_vortex2(g_eng, nt, mu); }
+int gpe_evolve_vortex3(int nt, double* mu) { return gpe_engine_evolve_vortex3(g_eng, nt, mu); }
+int gpe_evolve_enforced_phase(int nt, double* mu) { return gpe_engine_evolve_enforced_phase(g_eng, nt, mu); }
 int gpe_energy(double* t, double* ekin, double* eint, double* eext) {
     return gpe_engine_energy(g_eng, t, ekin, eint, eext);
 }

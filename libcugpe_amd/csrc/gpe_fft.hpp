@@ -318,7 +318,8 @@ enum PassFlags {
 template <typename T>
 struct PassArgs {
     int axis;              // 0 x, 1 y, 2 z
-    unsigned M;            // number of pencils
+    unsigned M;            // number of pencils (all instances)
+    unsigned n1;           // points per instance (instances are consecutive lattices)
     int dir1, dir2;        // -1 / +1; dir2 == 0: single transform
     int load_mode, mid_mode, store_mode, flags;
     int nred;              // number of FP64 sums produced (0..3)
@@ -341,6 +342,8 @@ struct PassArgs {
     const double* kz64;
     const double* scal;    // device scalars; scal[1] is the normalisation factor
     double* partials;      // [nred][gridDim.x]
+    int pin;               // phase pinning of psi0: 1 vortex, 2 vortex where |psi| > 1e-15, 3 stored phase
+    const double* phase;   // stored phase (pin 3)
     T kfac;                // MID_K2 / MID_KA factor
     int ka_axis;           // MID_KA axis
     double up64[kMaxUserParams];  // user params in FP64 (energy density)
@@ -351,28 +354,6 @@ struct Coord {
     unsigned off, ix, iy, iz;
 };
 
-template <typename T>
-__device__ __forceinline__ Coord coord(const PassArgs<T>& a, int N, unsigned p, unsigned i) {
-    Coord c;
-    const unsigned ny = (unsigned)a.ph.ny, nz = (unsigned)a.ph.nz;
-    if (a.axis == 2) {
-        c.ix = p / ny;
-        c.iy = p - c.ix * ny;
-        c.iz = i;
-        c.off = p * (unsigned)N + i;
-    } else if (a.axis == 1) {
-        c.ix = p / nz;
-        c.iz = p - c.ix * nz;
-        c.iy = i;
-        c.off = c.ix * (ny * nz) + c.iz + i * nz;
-    } else {
-        c.iy = p / nz;
-        c.iz = p - c.iy * nz;
-        c.ix = i;
-        c.off = p + i * (ny * nz);
-    }
-    return c;
-}
 
 // v * exp(re + i im).  In complex64 the factor's angle and modulus come from
 // single-precision sincos/exp, the direction is renormalised to exact unit
@@ -385,72 +366,37 @@ __device__ __forceinline__ cplx<float> mul_exp(cplx<float> v, float re, float im
     float s, c;
     sincosf(im, &s, &c);
     const double cd = (double)c, sd = (double)s;
-    const double f = (double)expf(re) / sqrt(cd * cd + sd * sd);
+    // 1/|(c, s)| to O((m-1)^2) ~ 1e-13, |m - 1| being a few float ulps
+    const double m = cd * cd + sd * sd;
+    double f = 1.5 - 0.5 * m;
+    if (re != 0.0f) f *= (double)expf(re);  // real time: re == 0 for every point
     const double vx = (double)v.x, vy = (double)v.y;
     return mk((float)(f * (vx * cd - vy * sd)), (float)(f * (vx * sd + vy * cd)));
 }
 
+
+// Phase pinning of the ITE variants: __gpe_imprint_vortexline_zdir_ (dev/gpe_engine.cu:485-511),
+// __gpe_imprint2_vortexline_zdir_ (520-548) and __gpe_enforce_phase__ (584-591)
 template <typename T>
-__device__ __forceinline__ T pot_at(const PassArgs<T>& a, const Coord& c, cplx<T> v, bool qf) {
-    T lv = vext(a.ph, c.ix, c.iy, c.iz, c.off) + dedfdn(a.ph, density(a.ph, v));
-    if (qf) lv = lv + a.vq[c.off];
-    return lv;
+__device__ __forceinline__ cplx<T> apply_pin(const PassArgs<T>& a, cplx<T> v, const Coord& c) {
+    if (a.pin == 1) return imprint(a.ph, v, c.ix, c.iy);
+    if (a.pin == 2) return (g_hypot(v.x, v.y) > (T)1e-15) ? imprint(a.ph, v, c.ix, c.iy) : v;
+    if (a.pin == 3) {
+        const T m = g_sqrt(v.x * v.x + v.y * v.y);
+        T sn, cs;
+        g_sincos((T)a.phase[c.off], &sn, &cs);
+        return mk(m * cs, m * sn);
+    }
+    return v;
 }
 
-// Vstep1: psi -> modify(psi) exp(c V(psi))  (orginal/gpe_engine.cu:723-772)
-template <typename T>
-__device__ __forceinline__ cplx<T> vstep1(const PassArgs<T>& a, const Coord& c, cplx<T> v, bool qf, T qval) {
-    T lv = vext(a.ph, c.ix, c.iy, c.iz, c.off) + dedfdn(a.ph, density(a.ph, v));
-    if (qf) lv = lv + qval;
-    return mul_exp(v, a.ph.c.x * lv, a.ph.c.y * lv);
-}
 
 __device__ __forceinline__ double dens64(double g, double x, double y) { return g * (x * x + y * y); }
 
 template <typename T>
-__device__ __forceinline__ cplx<T> load_op(const PassArgs<T>& a, const Coord& c, double* acc) {
-    const bool qf = (a.flags & F_QF) != 0;
-    if (a.load_mode == LOAD_PLAIN) {
-        cplx<T> v = a.src[c.off];
-        if (a.flags & F_RED_POT) {
-            const double rho = dens64((double)a.ph.gamma, (double)v.x, (double)v.y);
-            acc[0] += rho * (double)vext(a.ph, c.ix, c.iy, c.iz, c.off);
-            acc[1] += edf(a.ph.interactions, a.up64, rho);
-        }
-        return v;
-    }
-    if (a.load_mode == LOAD_PSI0) {
-        cplx<T> v = a.psiA[c.off];
-        bool wb = false;
-        if (a.flags & F_SCALE) { v = cscale(v, (T)a.scal[1]); wb = true; }
-        if (a.flags & F_IMPRINT) { v = imprint(a.ph, v, c.ix, c.iy); wb = true; }
-        if (wb) a.psiA[c.off] = v;
-        if (a.flags & F_VSTEP1) v = vstep1(a, c, v, qf, qf ? a.vq[c.off] : (T)0);
-        return v;
-    }
-    // LOAD_PART2: corrector potential from the (normalised) predictor
-    // (orginal/gpe_engine.cu:817-868)
-    cplx<T> phi3 = a.src[c.off];
-    if (a.flags & F_SCALE) phi3 = cscale(phi3, (T)a.scal[1]);
-    cplx<T> psi0 = a.psiA[c.off];
-    const T V0 = pot_at(a, c, psi0, qf);
-    T lv;
-    if (qf)
-        lv = (T)0.5 * (V0 + vext(a.ph, c.ix, c.iy, c.iz, c.off) + dedfdn(a.ph, density(a.ph, phi3)) + a.vq[c.off]);
-    else
-        lv = (T)0.5 * (V0 + vext(a.ph, c.ix, c.iy, c.iz, c.off) + dedfdn(a.ph, density(a.ph, phi3)));
-    a.vbuf[c.off] = lv;
-    if (a.flags & F_IMPRINT_MID) psi0 = imprint(a.ph, psi0, c.ix, c.iy);
-    return mul_exp(psi0, a.ph.c.x * lv, a.ph.c.y * lv);
-}
-
-template <typename T>
-__device__ __forceinline__ cplx<T> mid_op(const PassArgs<T>& a, const Coord& c, cplx<T> v, double* acc) {
-    const bool qf = (a.flags & F_QF) != 0;
+__device__ __forceinline__ cplx<T> mid_op_lean(const PassArgs<T>& a, const Coord& c, cplx<T> v) {
     switch (a.mid_mode) {
     case MID_EXPT: {
-        // orginal/gpe_engine.cu:880-895, in FP64 in both precisions: the factors
-        // are the same every step, so a rounded |K| != 1 would compound
         cplx<double> w = mk((double)v.x, (double)v.y);
         w = cmul(w, a.Kx[c.ix]);
         w = cmul(w, a.Ky[c.iy]);
@@ -465,59 +411,129 @@ __device__ __forceinline__ cplx<T> mid_op(const PassArgs<T>& a, const Coord& c, 
         const T k = (a.ka_axis == 0) ? a.kx[c.ix] : (a.ka_axis == 1) ? a.ky[c.iy] : a.kz[c.iz];
         return cscale(v, k / a.kfac);
     }
+    default:
+        return v;
+    }
+}
+
+// Inputs of one element of a physics pass, all taken from LDS (or registers)
+// by the caller: the lattice coordinates, the external potential there, psi0
+// and the stored real per-point field (V_qf or V1) when the pass needs them.
+template <typename T>
+struct ElemIn {
+    Coord c;
+    T vx;          // V_ext at the point (time independent hooks: the same at it and it+1)
+    cplx<T> aux;   // psi0
+    T raux;        // V_qf (F_QF) or V1 (MID_VSTEP3)
+};
+
+// V(psi) = Vext + dEDFdn(rho(psi)) [+ V_qf]   (orginal/gpe_engine.cu:739, 764)
+template <typename T>
+__device__ __forceinline__ T pot2(const PassArgs<T>& a, T vx, cplx<T> v, bool qf, T q) {
+    T lv = vx + dedfdn(a.ph, density(a.ph, v));
+    if (qf) lv = lv + q;
+    return lv;
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> load_op2(const PassArgs<T>& a, const ElemIn<T>& e, cplx<T> v, double* acc) {
+    const bool qf = (a.flags & F_QF) != 0;
+    if (a.load_mode == LOAD_PLAIN) {
+        if (a.flags & F_RED_POT) {
+            const double rho = dens64((double)a.ph.gamma, (double)v.x, (double)v.y);
+            acc[0] += rho * (double)e.vx;
+            acc[1] += edf(a.ph.interactions, a.up64, rho);
+        }
+        return v;
+    }
+    if (a.load_mode == LOAD_PSI0) {
+        // v is psi0 (orginal/gpe_engine.cu:723-772 after the rescale / imprint)
+        bool wb = false;
+        if (a.flags & F_SCALE) { v = cscale(v, (T)a.scal[1]); wb = true; }
+        if (a.flags & F_IMPRINT) { v = apply_pin(a, v, e.c); wb = true; }
+        if (wb) a.psiA[e.c.off] = v;
+        if (a.flags & F_VSTEP1) {
+            const T lv = pot2(a, e.vx, v, qf, e.raux);
+            v = mul_exp(v, a.ph.c.x * lv, a.ph.c.y * lv);
+        }
+        return v;
+    }
+    // LOAD_PART2: v is the predictor phi3 (orginal/gpe_engine.cu:817-868)
+    cplx<T> phi3 = v;
+    if (a.flags & F_SCALE) phi3 = cscale(phi3, (T)a.scal[1]);
+    cplx<T> psi0 = e.aux;
+    const T V0 = pot2(a, e.vx, psi0, qf, e.raux);
+    T lv;
+    if (qf) lv = (T)0.5 * (V0 + e.vx + dedfdn(a.ph, density(a.ph, phi3)) + e.raux);
+    else lv = (T)0.5 * (V0 + e.vx + dedfdn(a.ph, density(a.ph, phi3)));
+    a.vbuf[e.c.off] = lv;
+    if (a.flags & F_IMPRINT_MID) psi0 = apply_pin(a, psi0, e.c);
+    return mul_exp(psi0, a.ph.c.x * lv, a.ph.c.y * lv);
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> mid_op2(const PassArgs<T>& a, const ElemIn<T>& e, cplx<T> v, double* acc) {
+    const bool qf = (a.flags & F_QF) != 0;
+    switch (a.mid_mode) {
     case MID_VSTEP2: {
         // orginal/gpe_engine.cu:774-801 (and the _part1/_part2[_qf] pair for qf)
-        const cplx<T> psi0 = a.psiA[c.off];
-        const T V0 = pot_at(a, c, psi0, qf);
+        const cplx<T> psi0 = e.aux;
+        const T V0 = pot2(a, e.vx, psi0, qf, e.raux);
         const cplx<T> phi3 = mul_exp(v, a.ph.c.x * V0, a.ph.c.y * V0);
         T lv;
-        if (qf)
-            lv = (T)0.5 * (V0 + vext(a.ph, c.ix, c.iy, c.iz, c.off) + dedfdn(a.ph, density(a.ph, phi3)) + a.vq[c.off]);
-        else
-            lv = (T)0.5 * (V0 + vext(a.ph, c.ix, c.iy, c.iz, c.off) + dedfdn(a.ph, density(a.ph, phi3)));
-        a.vbuf[c.off] = lv;
+        if (qf) lv = (T)0.5 * (V0 + e.vx + dedfdn(a.ph, density(a.ph, phi3)) + e.raux);
+        else lv = (T)0.5 * (V0 + e.vx + dedfdn(a.ph, density(a.ph, phi3)));
+        a.vbuf[e.c.off] = lv;
         return mul_exp(psi0, a.ph.c.x * lv, a.ph.c.y * lv);
     }
     case MID_PART1: {
         // orginal/gpe_engine.cu:803-815 with the density sum of gpe_normalize fused
-        const cplx<T> psi0 = a.psiA[c.off];
-        const T V0 = pot_at(a, c, psi0, qf);
+        const T V0 = pot2(a, e.vx, e.aux, qf, e.raux);
         const cplx<T> phi3 = mul_exp(v, a.ph.c.x * V0, a.ph.c.y * V0);
         acc[0] += dens64((double)a.ph.gamma, (double)phi3.x, (double)phi3.y);
         return phi3;
     }
     case MID_VSTEP3: {
         // orginal/gpe_engine.cu:870-878; the factor exp(c V1) is recomputed from V1
-        const T lv = a.vbuf[c.off];
+        const T lv = e.raux;
         cplx<T> psi1 = mul_exp(v, a.ph.c.x * lv, a.ph.c.y * lv);
         if (a.flags & F_RED_NORM) acc[0] += dens64((double)a.ph.gamma, (double)psi1.x, (double)psi1.y);
         if (a.flags & (F_NEXT_VSTEP1 | F_NEXT_PSI)) {
-            if (a.flags & F_IMPRINT_NEXT) psi1 = imprint(a.ph, psi1, c.ix, c.iy);
-            a.psiA[c.off] = psi1;
-            if (a.flags & F_NEXT_VSTEP1) return vstep1(a, c, psi1, false, (T)0);
+            if (a.flags & F_IMPRINT_NEXT) psi1 = apply_pin(a, psi1, e.c);
+            a.psiA[e.c.off] = psi1;
+            if (a.flags & F_NEXT_VSTEP1) {
+                const T l1 = pot2(a, e.vx, psi1, false, (T)0);
+                return mul_exp(psi1, a.ph.c.x * l1, a.ph.c.y * l1);
+            }
             return psi1;
         }
-        a.psiA[c.off] = psi1;
+        a.psiA[e.c.off] = psi1;
         return psi1;
     }
     case MID_QF_VSTEP1: {
-        // V_qf = Im(conj psi0 * F^-1[qf' k^2 F psi0]) / (rho0 + eps), then Vstep1_qf
-        const cplx<T> psi0 = a.psiA[c.off];
+        // V_qf = Im(conj psi0 * F^-1[qf' k^2 F psi0]) / (rho0 + eps) (orginal 911-922), then Vstep1_qf
+        const cplx<T> psi0 = e.aux;
         const T r0 = density(a.ph, psi0);
         const T q = (psi0.x * v.y + (-psi0.y) * v.x) / (r0 + (T)1.0e-12);
-        a.vq[c.off] = q;
-        return vstep1(a, c, psi0, true, q);
+        a.vq[e.c.off] = q;
+        const T lv = pot2(a, e.vx, psi0, true, q);
+        return mul_exp(psi0, a.ph.c.x * lv, a.ph.c.y * lv);
     }
     case MID_KIN_SUM: {
-        const double k2 = a.kx64[c.ix] * a.kx64[c.ix] + a.ky64[c.iy] * a.ky64[c.iy] + a.kz64[c.iz] * a.kz64[c.iz];
+        const double k2 = a.kx64[e.c.ix] * a.kx64[e.c.ix] + a.ky64[e.c.iy] * a.ky64[e.c.iy] +
+                          a.kz64[e.c.iz] * a.kz64[e.c.iz];
         acc[0] += k2 * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
         return v;
     }
     case MID_OVERLAP_RE: {
-        const cplx<T> p = a.psiA[c.off];
-        a.rout[c.off] = (double)(p.x * v.x + p.y * v.y);
+        const cplx<T> p = e.aux;
+        a.rout[e.c.off] = (double)(p.x * v.x + p.y * v.y);
         return v;
     }
+    case MID_EXPT:
+    case MID_K2:
+    case MID_KA:
+        return mid_op_lean(a, e.c, v);
     default:
         return v;
     }
@@ -537,74 +553,521 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
     return s;
 }
 
-template <typename T, int N, bool CONTIG>
-__global__ __launch_bounds__(kThreads) void fft_pass_kernel(const PassArgs<T> a) {
+// ------------------------------------------------------ the pass kernel
+//
+// Each transform of a pass runs its Stockham stages as
+//   first stage   global memory -> butterfly registers -> LDS
+//   middle stages LDS -> LDS
+//   last stage    LDS -> butterfly registers
+// (a single-stage length goes global -> registers directly).  A second
+// transform in the other direction runs the reversed radix plan, whose first
+// stage takes the same elements per thread as the last stage of the first
+// transform, so the mid op and the hand-over stay in registers.  The final
+// registers go straight to global memory.
+
+constexpr FftPlan plan_of(int N, bool rev) {
+    FftPlan p = make_plan(N);
+    if (rev)
+        for (int i = 0; i < p.ns / 2; i++) {
+            int t = p.r[i];
+            p.r[i] = p.r[p.ns - 1 - i];
+            p.r[p.ns - 1 - i] = t;
+        }
+    return p;
+}
+
+template <typename T>
+__device__ __forceinline__ unsigned axis_stride(const PassArgs<T>& a) {
+    return a.axis == 2 ? 1u : a.axis == 1 ? (unsigned)a.ph.nz : (unsigned)(a.ph.ny * a.ph.nz);
+}
+
+// pencil slot -> global element offsets and lattice coordinates
+struct Pencil {
+    unsigned base;    // offset of element 0
+    unsigned stride;  // element stride along the axis
+    unsigned c0, c1;  // the two fixed coordinates
+};
+
+template <typename T>
+__device__ __forceinline__ Pencil make_pencil(const PassArgs<T>& a, int N, unsigned p) {
+    Pencil q;
+    const unsigned nx = (unsigned)a.ph.nx, ny = (unsigned)a.ph.ny, nz = (unsigned)a.ph.nz;
+    if (a.axis == 2) {  // z pencils (b, ix, iy) in order: p = (b nx + ix) ny + iy
+        const unsigned bx = p / ny;
+        q.c1 = p - bx * ny;
+        q.c0 = bx - (bx / nx) * nx;
+        q.base = p * (unsigned)N;
+        q.stride = 1;
+    } else if (a.axis == 1) {  // y pencils (b, ix, iz): p = (b nx + ix) nz + iz
+        const unsigned bx = p / nz;
+        q.c1 = p - bx * nz;
+        q.c0 = bx - (bx / nx) * nx;
+        q.base = bx * (ny * nz) + q.c1;
+        q.stride = nz;
+    } else {  // x pencils (b, iy, iz): p = b ny nz + iy nz + iz
+        const unsigned plane = ny * nz;
+        const unsigned b = p / plane;
+        const unsigned pl = p - b * plane;
+        q.c0 = pl / nz;
+        q.c1 = pl - q.c0 * nz;
+        q.base = b * a.n1 + pl;
+        q.stride = plane;
+    }
+    return q;
+}
+
+template <typename T>
+__device__ __forceinline__ Coord elem(const PassArgs<T>& a, const Pencil& q, unsigned i) {
+    Coord c;
+    c.off = q.base + i * q.stride;
+    if (a.axis == 2) { c.ix = q.c0; c.iy = q.c1; c.iz = i; }
+    else if (a.axis == 1) { c.ix = q.c0; c.iy = i; c.iz = q.c1; }
+    else { c.ix = i; c.iy = q.c0; c.iz = q.c1; }
+    return c;
+}
+
+// butterfly b of a stage with N/R = M butterflies per pencil -> (pencil slot, j)
+template <int M, int W, bool CONTIG>
+__device__ __forceinline__ void bmap(int b, int& w, int& j) {
+    if constexpr (CONTIG) { w = b / M; j = b % M; }
+    else { j = b / W; w = b % W; }
+}
+
+// first stage from global memory (load op applied per element) into LDS
+template <typename T, int N, int W, bool CONTIG, bool FULL, int S, int R>
+__device__ __forceinline__ void first_stage_global(const PassArgs<T>& a, unsigned p0, cplx<T>* lds, double* acc) {
+    constexpr int M = N / R;
+    constexpr int NB = W * M;
+    constexpr int PER = (NB + kThreads - 1) / kThreads;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int b = threadIdx.x + q * kThreads;
+        if ((NB % kThreads) == 0 || b < NB) {
+            int w, j;
+            bmap<M, W, CONTIG>(b, w, j);
+            const unsigned p = p0 + (unsigned)w;
+            cplx<T> v[R];
+            if (p < a.M) {
+                const Pencil pc = make_pencil(a, N, p);
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const Coord c = elem(a, pc, (unsigned)(j + r * M));
+                    v[r] = a.src[c.off];
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) v[r] = mk((T)0, (T)0);
+            }
+            dft<R, S>(v);
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[lidx<N, W, CONTIG>(w, j * R + r)] = v[r];
+        }
+    }
+}
+
+// first stage from registers already holding (w, j + r N/R) into LDS
+template <typename T, int N, int W, bool CONTIG, int S, int R, int PER>
+__device__ __forceinline__ void first_stage_regs(cplx<T> (&v)[PER][R], cplx<T>* lds) {
+    constexpr int M = N / R;
+    constexpr int NB = W * M;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int b = threadIdx.x + q * kThreads;
+        if ((NB % kThreads) == 0 || b < NB) {
+            int w, j;
+            bmap<M, W, CONTIG>(b, w, j);
+            dft<R, S>(v[q]);
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[lidx<N, W, CONTIG>(w, j * R + r)] = v[q][r];
+        }
+    }
+}
+
+// stages [STAGE, LAST) of plan (N, REV) in LDS
+template <typename T, int N, int W, bool CONTIG, int S, bool REV, int STAGE, int LAST>
+__device__ __forceinline__ void lds_stages(cplx<T>* lds, const cplx<T>* tw, const cplx<T>* twlo) {
+    if constexpr (STAGE < LAST) {
+        constexpr FftPlan P = plan_of(N, REV);
+        fft_stage<T, N, W, CONTIG, S, P.r[STAGE], plan_span(P, STAGE)>(lds, tw, twlo);
+        lds_stages<T, N, W, CONTIG, S, REV, STAGE + 1, LAST>(lds, tw, twlo);
+    }
+}
+
+// last stage (span N/R) from LDS into registers: v[q][r] is element (w, j + r N/R)
+template <typename T, int N, int W, bool CONTIG, int S, int R, int PER>
+__device__ __forceinline__ void last_stage_regs(const cplx<T>* lds, const cplx<T>* tw, const cplx<T>* twlo,
+                                                cplx<T> (&v)[PER][R]) {
+    constexpr int M = N / R;
+    constexpr int NB = W * M;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int b = threadIdx.x + q * kThreads;
+        if ((NB % kThreads) == 0 || b < NB) {
+            int w, j;
+            bmap<M, W, CONTIG>(b, w, j);
+#pragma unroll
+            for (int r = 0; r < R; r++) v[q][r] = lds[lidx<N, W, CONTIG>(w, j + r * M)];
+#pragma unroll
+            for (int r = 1; r < R; r++) {  // twiddle w_N^(j r)
+                cplx<T> t = tw[j * r];
+                if (S > 0) t.y = -t.y;
+                if constexpr (sizeof(T) == 4) {
+                    cplx<T> tl = twlo[j * r];
+                    if (S > 0) tl.y = -tl.y;
+                    v[q][r] = cmul_comp(v[q][r], t, tl);
+                } else {
+                    v[q][r] = cmul(v[q][r], t);
+                }
+            }
+            dft<R, S>(v[q]);
+        }
+    }
+}
+
+// registers (w, j + r N/R) <- global memory with the load op (single-stage lengths)
+template <typename T, int N, int W, bool CONTIG, bool FULL, int R, int PER>
+__device__ __forceinline__ void load_regs(const PassArgs<T>& a, unsigned p0, cplx<T> (&v)[PER][R], double* acc) {
+    constexpr int M = N / R;
+    constexpr int NB = W * M;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int b = threadIdx.x + q * kThreads;
+        int w, j;
+        bmap<M, W, CONTIG>(b, w, j);
+        const unsigned p = p0 + (unsigned)w;
+        const bool ok = ((NB % kThreads) == 0 || b < NB) && p < a.M;
+        const Pencil pc = make_pencil(a, N, ok ? p : 0u);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (ok) {
+                const Coord c = elem(a, pc, (unsigned)(j + r * M));
+                v[q][r] = a.src[c.off];
+            } else {
+                v[q][r] = mk((T)0, (T)0);
+            }
+        }
+    }
+}
+
+template <typename T, int N, int W, bool CONTIG, bool FULL, int R, int PER>
+__device__ __forceinline__ void mid_regs(const PassArgs<T>& a, unsigned p0, cplx<T> (&v)[PER][R], double* acc) {
+    constexpr int M = N / R;
+    constexpr int NB = W * M;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int b = threadIdx.x + q * kThreads;
+        int w, j;
+        bmap<M, W, CONTIG>(b, w, j);
+        const unsigned p = p0 + (unsigned)w;
+        if (((NB % kThreads) == 0 || b < NB) && p < a.M) {
+            const Pencil pc = make_pencil(a, N, p);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const Coord c = elem(a, pc, (unsigned)(j + r * M));
+                v[q][r] = mid_op_lean(a, c, v[q][r]);
+            }
+        }
+    }
+}
+
+template <typename T, int N, int W, bool CONTIG, int R, int PER>
+__device__ __forceinline__ void store_regs(const PassArgs<T>& a, unsigned p0, const cplx<T> (&v)[PER][R]) {
+    constexpr int M = N / R;
+    constexpr int NB = W * M;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int b = threadIdx.x + q * kThreads;
+        int w, j;
+        bmap<M, W, CONTIG>(b, w, j);
+        const unsigned p = p0 + (unsigned)w;
+        if (((NB % kThreads) == 0 || b < NB) && p < a.M) {
+            const Pencil pc = make_pencil(a, N, p);
+#pragma unroll
+            for (int r = 0; r < R; r++) a.dst[pc.base + (unsigned)(j + r * M) * pc.stride] = v[q][r];
+        }
+    }
+}
+
+// pass shapes: one transform forward / inverse, or two with the mid op between
+enum PassShape { SHAPE_F = 0, SHAPE_I = 1, SHAPE_FI = 2, SHAPE_IF = 3 };
+
+// Lean passes (plain load and store, at most a table factor in the middle):
+// everything between the first load and the last store stays in registers or LDS.
+template <typename T, int N, bool CONTIG, int S1, int S2>
+__device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, const cplx<T>* tw,
+                                          const cplx<T>* twlo) {
     constexpr int W = tile_w<T, N>();
+    constexpr FftPlan P = plan_of(N, false);
+    constexpr int NSG = P.ns;
+    constexpr int RL = P.r[NSG - 1];
+    constexpr int PERL = (W * (N / RL) + kThreads - 1) / kThreads;
+    const unsigned p0 = blockIdx.x * (unsigned)W;
+    cplx<T> v[PERL][RL];
+    if constexpr (NSG == 1) {
+        load_regs<T, N, W, CONTIG, false, RL, PERL>(a, p0, v, nullptr);
+#pragma unroll
+        for (int q = 0; q < PERL; q++) dft<RL, S1>(v[q]);
+    } else {
+        first_stage_global<T, N, W, CONTIG, false, S1, P.r[0]>(a, p0, lds, nullptr);
+        __syncthreads();
+        lds_stages<T, N, W, CONTIG, S1, false, 1, NSG - 1>(lds, tw, twlo);
+        last_stage_regs<T, N, W, CONTIG, S1, RL, PERL>(lds, tw, twlo, v);
+    }
+    if (a.mid_mode != MID_NONE) mid_regs<T, N, W, CONTIG, false, RL, PERL>(a, p0, v, nullptr);
+    if constexpr (S2 != 0) {
+        if constexpr (NSG == 1) {
+#pragma unroll
+            for (int q = 0; q < PERL; q++) dft<RL, S2>(v[q]);
+        } else {
+            constexpr FftPlan Q = plan_of(N, true);
+            static_assert(Q.r[0] == RL, "reversed plan must start with the last radix");
+            constexpr int RF = Q.r[NSG - 1];
+            constexpr int PERF = (W * (N / RF) + kThreads - 1) / kThreads;
+            __syncthreads();  // the first transform's last-stage reads are done
+            first_stage_regs<T, N, W, CONTIG, S2, RL, PERL>(v, lds);
+            __syncthreads();
+            lds_stages<T, N, W, CONTIG, S2, true, 1, NSG - 1>(lds, tw, twlo);
+            cplx<T> u[PERF][RF];
+            last_stage_regs<T, N, W, CONTIG, S2, RF, PERF>(lds, tw, twlo, u);
+            store_regs<T, N, W, CONTIG, RF, PERF>(a, p0, u);
+            return;
+        }
+    }
+    store_regs<T, N, W, CONTIG, RL, PERL>(a, p0, v);
+}
+
+// Physics passes.  Every per-element global input is gathered into LDS up
+// front by unrolled plain loads (the FFT input; psi0 as a complex side tile;
+// V_qf or V1 as a real side tile), together with per-pencil coordinates and the
+// separable external potential (V_xy per pencil, V_z per point).  The load and
+// mid ops then read only LDS, element by element in rolled loops (the op code
+// is not replicated per register), and their global outputs are plain stores.
+// The transforms run in LDS, the last stage into registers for the store.
+
+template <typename T, int N, bool CONTIG>
+constexpr int full_tile_w() {
+    int w = tile_w<T, N>();
+    if (CONTIG && w > 1) w /= 2;  // three tiles of LDS per workgroup
+    return w;
+}
+
+template <typename T, int N, bool CONTIG>
+struct FullLds {
+    static constexpr int W = full_tile_w<T, N, CONTIG>();
+    static constexpr int TILE = lds_elems<N, W, CONTIG>();
+    static constexpr int SIDE = CONTIG ? TILE : 1;   // side tiles only in z passes
+    cplx<T> main[TILE];
+    cplx<T> aux[SIDE];
+    T raux[SIDE];
+    T vz[CONTIG ? N : 1];
+    T vxy[W];
+    unsigned base[W], c0[W], c1[W];
+    bool ok[W];
+};
+
+template <typename T>
+__device__ __forceinline__ bool needs_psi0(const PassArgs<T>& a) {
+    return a.load_mode == LOAD_PART2 || a.mid_mode == MID_VSTEP2 || a.mid_mode == MID_PART1 ||
+           a.mid_mode == MID_QF_VSTEP1 || a.mid_mode == MID_OVERLAP_RE;
+}
+template <typename T>
+__device__ __forceinline__ const T* real_side(const PassArgs<T>& a) {
+    if (a.mid_mode == MID_VSTEP3) return a.vbuf;
+    const bool qf = (a.flags & F_QF) != 0;
+    if (qf && (a.load_mode == LOAD_PART2 || a.mid_mode == MID_VSTEP2 || a.mid_mode == MID_PART1 ||
+               (a.load_mode == LOAD_PSI0 && (a.flags & F_VSTEP1))))
+        return a.vq;
+    return nullptr;
+}
+
+template <typename T, int N, bool CONTIG>
+__device__ __forceinline__ ElemIn<T> elem_in(const PassArgs<T>& a, const FullLds<T, N, CONTIG>& L, int w, int i,
+                                             bool sep, T vscale) {
+    ElemIn<T> e;
+    const unsigned ui = (unsigned)i;
+    if (a.axis == 2) { e.c.ix = L.c0[w]; e.c.iy = L.c1[w]; e.c.iz = ui; e.c.off = L.base[w] + ui; }
+    else if (a.axis == 1) { e.c.ix = L.c0[w]; e.c.iy = ui; e.c.iz = L.c1[w]; e.c.off = L.base[w] + ui * (unsigned)a.ph.nz; }
+    else { e.c.ix = ui; e.c.iy = L.c0[w]; e.c.iz = L.c1[w]; e.c.off = L.base[w] + ui * (unsigned)(a.ph.ny * a.ph.nz); }
+    if constexpr (CONTIG) {
+        e.vx = sep ? vscale * (L.vxy[w] + L.vz[i]) : vext(a.ph, e.c.ix, e.c.iy, e.c.iz, e.c.off);
+        const int li = lidx<N, FullLds<T, N, CONTIG>::W, CONTIG>(w, i);
+        e.aux = L.aux[li];
+        e.raux = L.raux[li];
+    } else {
+        e.vx = (T)0;
+        e.aux = mk((T)0, (T)0);
+        e.raux = (T)0;
+    }
+    return e;
+}
+
+template <typename T, int N, bool CONTIG, int S1, int S2>
+__device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CONTIG>& L, const cplx<T>* tw,
+                                          const cplx<T>* twlo, double* acc) {
+    constexpr int W = FullLds<T, N, CONTIG>::W;
     constexpr int TILE = W * N;
     constexpr int PERE = (TILE + kThreads - 1) / kThreads;
-    __shared__ cplx<T> lds[lds_elems<N, W, CONTIG>()];
-    __shared__ cplx<T> tw[N];
-    __shared__ cplx<T> twlo[sizeof(T) == 4 ? N : 1];
-    __shared__ double red[kThreads / 64];
-
-    for (int i = threadIdx.x; i < N; i += kThreads) {
-        tw[i] = a.tw[i];
-        if constexpr (sizeof(T) == 4) twlo[i] = a.twlo[i];
-    }
+    constexpr FftPlan P = plan_of(N, false);
+    constexpr int NSG = P.ns;
+    constexpr int RL = P.r[NSG - 1];
+    constexpr int PERL = (W * (N / RL) + kThreads - 1) / kThreads;
     const unsigned p0 = blockIdx.x * (unsigned)W;
-    double acc[3] = {0.0, 0.0, 0.0};
 
-for (int q = 0; q < PERE; q++) {
+    // per-pencil coordinates and the pencil part of a separable potential
+    const int vk = a.ph.vext;
+    const bool sep = CONTIG && (vk == VEXT_HARMONIC || vk == VEXT_COSINE || vk == VEXT_WOODS_SAXON);
+    const T vscale = (vk == VEXT_COSINE) ? a.ph.up[0] : (T)1;
+    if (threadIdx.x < W) {
+        const int w = threadIdx.x;
+        const unsigned p = p0 + (unsigned)w;
+        L.ok[w] = p < a.M;
+        const Pencil pc = make_pencil(a, N, L.ok[w] ? p : 0u);
+        L.base[w] = pc.base;
+        L.c0[w] = pc.c0;
+        L.c1[w] = pc.c1;
+        if (sep) {
+            const int nx = a.ph.nx, ny = a.ph.ny;
+            if (vk == VEXT_WOODS_SAXON) L.vxy[w] = vext(a.ph, pc.c0, pc.c1, 0u, 0u);
+            else L.vxy[w] = a.ph.vtab[pc.c0] + a.ph.vtab[nx + pc.c1];
+            (void)ny;
+        }
+    }
+    if constexpr (CONTIG) {
+        if (sep)
+            for (int i = threadIdx.x; i < N; i += kThreads)
+                L.vz[i] = (vk == VEXT_WOODS_SAXON) ? (T)0 : a.ph.vtab[a.ph.nx + a.ph.ny + i];
+    }
+
+    // gather: unrolled plain loads of every input of the tile
+    const cplx<T>* msrc = (a.load_mode == LOAD_PSI0) ? a.psiA : a.src;
+    const bool want_psi0 = CONTIG && needs_psi0(a);
+    const T* rsrc = CONTIG ? real_side(a) : nullptr;
+#pragma unroll
+    for (int q = 0; q < PERE; q++) {
         const int e = threadIdx.x + q * kThreads;
         if ((TILE % kThreads) == 0 || e < TILE) {
             int w, i;
             if constexpr (CONTIG) { w = e / N; i = e % N; }
             else { i = e / W; w = e % W; }
             const unsigned p = p0 + (unsigned)w;
-            cplx<T> v = mk((T)0, (T)0);
-            if (p < a.M) v = load_op(a, coord(a, N, p, (unsigned)i), acc);
-            lds[lidx<N, W, CONTIG>(w, i)] = v;
+            const int li = lidx<N, W, CONTIG>(w, i);
+            if (p < a.M) {
+                const Pencil pc = make_pencil(a, N, p);
+                const unsigned off = pc.base + (unsigned)i * pc.stride;
+                L.main[li] = msrc[off];
+                if constexpr (CONTIG) {
+                    if (want_psi0) L.aux[li] = a.psiA[off];
+                    if (rsrc) L.raux[li] = rsrc[off];
+                }
+            } else {
+                L.main[li] = mk((T)0, (T)0);
+            }
         }
     }
     __syncthreads();
-    if (a.dir1 < 0) fft_stages<T, N, W, CONTIG, -1, 0>(lds, tw, twlo);
-    else fft_stages<T, N, W, CONTIG, 1, 0>(lds, tw, twlo);
 
-    if (a.mid_mode != MID_NONE) {
-for (int q = 0; q < PERE; q++) {
+    // load op, element by element from LDS
+    if (a.load_mode != LOAD_PLAIN || (a.flags & F_RED_POT)) {
+        for (int q = 0; q < PERE; q++) {
             const int e = threadIdx.x + q * kThreads;
             if ((TILE % kThreads) == 0 || e < TILE) {
                 int w, i;
                 if constexpr (CONTIG) { w = e / N; i = e % N; }
                 else { i = e / W; w = e % W; }
-                const unsigned p = p0 + (unsigned)w;
-                if (p < a.M) {
+                if (L.ok[w]) {
                     const int li = lidx<N, W, CONTIG>(w, i);
-                    lds[li] = mid_op(a, coord(a, N, p, (unsigned)i), lds[li], acc);
+                    L.main[li] = load_op2(a, elem_in(a, L, w, i, sep, vscale), L.main[li], acc);
                 }
             }
         }
         __syncthreads();
     }
-    if (a.dir2 < 0) fft_stages<T, N, W, CONTIG, -1, 0>(lds, tw, twlo);
-    else if (a.dir2 > 0) fft_stages<T, N, W, CONTIG, 1, 0>(lds, tw, twlo);
 
-    if (a.store_mode == STORE_PLAIN) {
-for (int q = 0; q < PERE; q++) {
+    cplx<T> v[PERL][RL];
+    if (a.mid_mode == MID_NONE) {
+        lds_stages<T, N, W, CONTIG, S1, false, 0, NSG - 1>(L.main, tw, twlo);
+        last_stage_regs<T, N, W, CONTIG, S1, RL, PERL>(L.main, tw, twlo, v);
+        if (a.store_mode == STORE_PLAIN) store_regs<T, N, W, CONTIG, RL, PERL>(a, p0, v);
+        return;
+    }
+    lds_stages<T, N, W, CONTIG, S1, false, 0, NSG>(L.main, tw, twlo);
+    for (int q = 0; q < PERE; q++) {  // mid op, natural order
+        const int e = threadIdx.x + q * kThreads;
+        if ((TILE % kThreads) == 0 || e < TILE) {
+            int w, i;
+            if constexpr (CONTIG) { w = e / N; i = e % N; }
+            else { i = e / W; w = e % W; }
+            if (L.ok[w]) {
+                const int li = lidx<N, W, CONTIG>(w, i);
+                L.main[li] = mid_op2(a, elem_in(a, L, w, i, sep, vscale), L.main[li], acc);
+            }
+        }
+    }
+    __syncthreads();
+    if constexpr (S2 != 0) {
+        lds_stages<T, N, W, CONTIG, S2, false, 0, NSG - 1>(L.main, tw, twlo);
+        last_stage_regs<T, N, W, CONTIG, S2, RL, PERL>(L.main, tw, twlo, v);
+        if (a.store_mode == STORE_PLAIN) store_regs<T, N, W, CONTIG, RL, PERL>(a, p0, v);
+    } else if (a.store_mode == STORE_PLAIN) {
+        for (int q = 0; q < PERE; q++) {
             const int e = threadIdx.x + q * kThreads;
             if ((TILE % kThreads) == 0 || e < TILE) {
                 int w, i;
                 if constexpr (CONTIG) { w = e / N; i = e % N; }
                 else { i = e / W; w = e % W; }
-                const unsigned p = p0 + (unsigned)w;
-                if (p < a.M) a.dst[coord(a, N, p, (unsigned)i).off] = lds[lidx<N, W, CONTIG>(w, i)];
+                if (L.ok[w]) a.dst[L.base[w] + (unsigned)i * axis_stride(a)] = L.main[lidx<N, W, CONTIG>(w, i)];
             }
         }
     }
-    for (int r = 0; r < a.nred; r++) {
-        const double s = block_sum(acc[r], red);
-        if (threadIdx.x == 0) a.partials[(size_t)r * gridDim.x + blockIdx.x] = s;
+}
+
+template <typename T, int N, bool CONTIG, bool FULL, int SHAPE>
+__global__ __launch_bounds__(kThreads) void fft_pass_kernel(const PassArgs<T> a) {
+    __shared__ cplx<T> tw[N];
+    __shared__ cplx<T> twlo[sizeof(T) == 4 ? N : 1];
+    for (int i = threadIdx.x; i < N; i += kThreads) {
+        tw[i] = a.tw[i];
+        if constexpr (sizeof(T) == 4) twlo[i] = a.twlo[i];
     }
+    // (the first barrier of every body orders these table writes before any use;
+    // a single-stage length reads no table)
+    constexpr int S1 = (SHAPE == SHAPE_F || SHAPE == SHAPE_FI) ? -1 : 1;
+    constexpr int S2 = (SHAPE == SHAPE_FI) ? 1 : (SHAPE == SHAPE_IF) ? -1 : 0;
+    if constexpr (!FULL) {
+        constexpr int W = tile_w<T, N>();
+        __shared__ cplx<T> lds[lds_elems<N, W, CONTIG>()];
+        lean_body<T, N, CONTIG, S1, S2>(a, lds, tw, twlo);
+    } else {
+        __shared__ FullLds<T, N, CONTIG> L;
+        __shared__ double red[kThreads / 64];
+        double acc[3] = {0.0, 0.0, 0.0};
+        full_body<T, N, CONTIG, S1, S2>(a, L, tw, twlo, acc);
+        // FP64 partial sums of this workgroup (indices fixed at compile time)
+        if (a.nred > 0) {
+            const double s0 = block_sum(acc[0], red);
+            if (threadIdx.x == 0) a.partials[blockIdx.x] = s0;
+        }
+        if (a.nred > 1) {
+            const double s1 = block_sum(acc[1], red);
+            if (threadIdx.x == 0) a.partials[(size_t)gridDim.x + blockIdx.x] = s1;
+        }
+        if (a.nred > 2) {
+            const double s2 = block_sum(acc[2], red);
+            if (threadIdx.x == 0) a.partials[2 * (size_t)gridDim.x + blockIdx.x] = s2;
+        }
+    }
+}
+
+// the pass can run in the lean kernel
+template <typename T>
+inline bool pass_is_lean(const PassArgs<T>& a) {
+    return a.load_mode == LOAD_PLAIN && a.flags == 0 && a.nred == 0 &&
+           (a.mid_mode == MID_NONE || a.mid_mode == MID_EXPT || a.mid_mode == MID_K2 || a.mid_mode == MID_KA);
+}
+
+inline int pass_shape(int dir1, int dir2) {
+    if (dir2 == 0) return dir1 < 0 ? SHAPE_F : SHAPE_I;
+    return dir1 < 0 ? SHAPE_FI : SHAPE_IF;
 }
 
 }  // namespace gpe

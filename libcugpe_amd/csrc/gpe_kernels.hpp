@@ -65,6 +65,9 @@ void launch_from_c128(const cplx<double>* in, cplx<T>* out, unsigned n, hipStrea
 template <typename T>
 void launch_to_c128(const cplx<T>* in, cplx<double>* out, unsigned n, hipStream_t s);
 
+// plain device-to-device copy kernel of `bytes` (a multiple of 16)
+void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+
 constexpr int kMaxReduceBlocks = 1024;
 
 }  // namespace gpe

@@ -27,6 +27,21 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(const double* __rest
 
 bool fft_length_supported(int N) { return fft_tile_width<double>(N) > 0; }
 
+// plain streaming copy (16-byte vectors); reference point for hardware byte counters
+__global__ __launch_bounds__(kThreads) void copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                                                        size_t n16) {
+    const size_t stride = (size_t)gridDim.x * kThreads;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    const size_t n16 = bytes / 16;
+    size_t blocks = (n16 + kThreads - 1) / kThreads;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, (uint4*)dst, (const uint4*)src, n16);
+}
+
 void launch_finalize(const double* partials, int nb, int nq, double* scal, int mode, int slot, double npart,
                      double dt, double div, hipStream_t s) {
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, s, partials, nb, nq, scal, mode, slot, npart,

@@ -21,7 +21,7 @@ unsigned launch_fft_pass_impl(int N, const PassArgs<T>& a, hipStream_t s) {
 template <typename T>
 int fft_tile_width_impl(int N) {
 #define GPE_CASE(NN) \
-    case NN: return tile_w<T, NN>();
+    case NN: return full_tile_w<T, NN, true>();  /* the narrowest tile of any pass */
     switch (N) {
         GPE_FFT_LENGTHS(GPE_CASE)
     default:

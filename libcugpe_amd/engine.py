@@ -62,6 +62,9 @@ def lib():
             "gpe_engine_get_norm": [_p, _dp],
             "gpe_engine_evolve_qf": [_p, _i, _dp],
             "gpe_engine_evolve_vortex": [_p, _i, _dp],
+            "gpe_engine_evolve_vortex2": [_p, _i, _dp],
+            "gpe_engine_evolve_vortex3": [_p, _i, _dp],
+            "gpe_engine_evolve_enforced_phase": [_p, _i, _dp],
             "gpe_engine_energy": [_p, _dp, _dp, _dp, _dp],
             "gpe_engine_get_density": [_p, _dp, _p],
             "gpe_engine_get_currents": [_p, _dp, _p, _p, _p],
@@ -76,6 +79,7 @@ def lib():
             "gpe_engine_synchronize": [_p],
             "gpe_engine_profile": [_p, _i, _i],
             "gpe_engine_profile_read": [_p, _dp, ctypes.POINTER(ctypes.c_long)],
+            "gpe_engine_copy_benchmark": [_p, _dp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -192,6 +196,20 @@ class Engine:
         _check(lib().gpe_engine_evolve_vortex(self._h, nt, ctypes.byref(mu) if want_mu else None), "gpe_evolve_vortex")
         return mu.value if want_mu else None
 
+    def _evolve_variant(self, name, nt, want_mu):
+        mu = ctypes.c_double(float("nan"))
+        _check(getattr(lib(), name)(self._h, nt, ctypes.byref(mu) if want_mu else None), name)
+        return mu.value if want_mu else None
+
+    def evolve_vortex2(self, nt, want_mu=False):
+        return self._evolve_variant("gpe_engine_evolve_vortex2", nt, want_mu)
+
+    def evolve_vortex3(self, nt, want_mu=False):
+        return self._evolve_variant("gpe_engine_evolve_vortex3", nt, want_mu)
+
+    def evolve_enforced_phase(self, nt, want_mu=False):
+        return self._evolve_variant("gpe_engine_evolve_enforced_phase", nt, want_mu)
+
     # -------------------------------------------------------- observables
     def energy(self):
         t, ek, ei, ee = (ctypes.c_double() for _ in range(4))
@@ -252,6 +270,12 @@ class Engine:
 
     def profile(self, axis_class, enable=True):
         _check(lib().gpe_engine_profile(self._h, axis_class, 1 if enable else 0), "gpe_engine_profile")
+
+    def copy_benchmark(self):
+        """Milliseconds of one plain copy kernel of the state array."""
+        ms = ctypes.c_double()
+        _check(lib().gpe_engine_copy_benchmark(self._h, ctypes.byref(ms)), "gpe_engine_copy_benchmark")
+        return ms.value
 
     def profile_read(self):
         ms = ctypes.c_double()

@@ -207,24 +207,35 @@ def test_windings_bit_exact(oracle, prec):
     g.close()
 
 
-def test_c64_energy_error_vs_fp64(oracle):
-    """complex64 engine vs FP64 oracle after 200 real-time steps on 64^3."""
-    say("c64 accuracy")
+@pytest.mark.parametrize("state,steps,tol", [("vortex", 500, 1e-6), ("random", 200, 2e-6)])
+def test_c64_energy_error_vs_fp64(oracle, state, steps, tol):
+    """complex64 engine vs FP64 oracle in real time on 64^3.  The vortex state
+    is the bench workload's (near-uniform density, 20 vortex lines, cosine
+    potential); the random-phase state (phase noise of 0.5 rad per site, so
+    |k| up to pi everywhere) is the harsh case and carries a looser bound."""
+    say("c64 accuracy", state)
+    import bench
+
     shape = (64, 64, 64)
     params = [0.1, 2.0, 0.0, 0.008]
     g, o = pair(oracle, shape, 1.0, 0.0, 0.02, float(np.prod(shape)), E.CONTACT, E.COSINE, params, precision=E.C64)
-    rng = np.random.default_rng(2)
-    psi = np.exp(1j * 0.5 * rng.standard_normal(shape)) * (1 + 0.01 * rng.random(shape))
+    if state == "random":
+        rng = np.random.default_rng(2)
+        psi = np.exp(1j * 0.5 * rng.standard_normal(shape)) * (1 + 0.01 * rng.random(shape))
+    else:
+        psi = bench.initial_state(shape)
     for x in (g, o):
         x.set_psi(psi)
         x.normalize_psi()
-    g.evolve_qf(200)
-    o.evolve_qf(200)
+    g.evolve_qf(steps)
+    o.evolve_qf(steps)
+    ng, no = g.get_norm(), o.get_norm()
     eg, eo = g.energy(), o.energy()
     Eg, Eo = sum(eg[1:]), sum(eo[1:])
     rel = abs(Eg - Eo) / abs(Eo)
-    say("c64 rel energy error after 200 steps: %.3e" % rel)
-    assert rel < 1e-6
+    say("c64 %s: rel energy error after %d steps %.3e, norm drift %.2e" % (state, steps, rel, ng / no - 1))
+    assert rel < tol
+    assert abs(ng / no - 1) < 5e-6
     dg, _ = g.get_density()
     do, _ = o.get_density()
     assert np.max(np.abs(dg - do)) / np.max(do) < 1e-3

@@ -243,3 +243,46 @@ def test_vortex_imprint_phase_and_modulus(oracle):
     # dt -> 0 with no potential: one vortex step is the imprint up to O(dt)
     exp_phase = np.exp(1j * ph)[:, :, None]
     assert np.max(np.abs(np.angle(out / (np.abs(out) * exp_phase)))) < 1e-6
+
+
+def test_vortex_variants_closed_forms(oracle):
+    # dev/gpe_engine.cu:1115-1477.  In real time the mid-step re-imprint does not
+    # exist, so gpe_evolve_vortex3 and gpe_evolve_vortex are the same map.
+    shape = (12, 10, 6)
+    rng = np.random.default_rng(14)
+    psi = (rng.random(shape) + 0.2) * np.exp(0.3j * rng.standard_normal(shape))
+    a = oracle.OracleEngine(shape, 1.0, 0.0, 0.05, 1.0, CONTACT, HARMONIC, 1)
+    b = oracle.OracleEngine(shape, 1.0, 0.0, 0.05, 1.0, CONTACT, HARMONIC, 1)
+    for e in (a, b):
+        e.set_user_params([0.1, 0.2, 0.3, 0.01])
+        e.set_vortex(0.5, -0.5, 1)
+        e.set_psi(psi)
+    a.evolve_vortex(3)
+    b.evolve_vortex3(3)
+    assert np.array_equal(a.get_psi()[0], b.get_psi()[0])
+    # dt -> 0: vortex2 imprints only where |psi| > 1e-15, zeros stay zero
+    z = psi.copy()
+    z[3, 4, :] = 0.0
+    c = oracle.OracleEngine(shape, 0.0, 1.0, 1e-12, 1.0, CONTACT, HARMONIC, 1)
+    c.set_user_params([0.0, 0.0, 0.0, 0.0])
+    c.set_vortex(0.5, -0.5, 2)
+    c.set_psi(z)
+    c.evolve_vortex2(1)
+    out, _ = c.get_psi()
+    assert np.max(np.abs(out[3, 4, :])) < 1e-12
+    ix, iy = np.meshgrid(np.arange(12), np.arange(10), indexing="ij")
+    ph = np.exp(2j * np.arctan2(ix - 6 - 0.5, iy - 5 + 0.5))[:, :, None]
+    mask = np.abs(out) > 1e-6
+    assert np.max(np.abs(np.angle(out[mask] / (np.abs(out[mask]) * np.broadcast_to(ph, shape)[mask])))) < 1e-6
+    # dt -> 0: the enforced phase is the stored one, the modulus is kept
+    d = oracle.OracleEngine(shape, 0.0, 1.0, 1e-12, 1.0, CONTACT, HARMONIC, 1)
+    d.set_user_params([0.0, 0.0, 0.0, 0.0])
+    d.set_psi(psi)
+    target = rng.uniform(-np.pi, np.pi, shape)
+    d.set_phase(target)
+    d.evolve_enforced_phase(1)
+    out, _ = d.get_psi()
+    assert np.max(np.abs(np.angle(out * np.exp(-1j * target)))) < 1e-9
+    assert np.max(np.abs(np.abs(out) - np.abs(psi) * np.sqrt(1.0 / np.sum(np.abs(psi) ** 2)))) < 1e-6
+    # get_phase returns arg(psi) and keeps it as the enforced phase
+    assert np.allclose(d.get_phase(), np.angle(out), atol=1e-15)
