@@ -67,6 +67,7 @@ struct gpe_engine_s {
     void *tw[3] = {nullptr, nullptr, nullptr};        // per-axis twiddles
     void *twlo[3] = {nullptr, nullptr, nullptr};      // complex64: low parts of the twiddles
     void *Ktab = nullptr, *ktab = nullptr;           // kinetic factors / wave numbers
+    void *Khl = nullptr;                             // complex64: K_x and K_y K_z as hi + lo float pairs
     void *vtab = nullptr;                            // per-axis terms of separable potentials
     double *k64 = nullptr;
     double *partials = nullptr;
@@ -143,13 +144,28 @@ void build_kinetic(gpe_engine_s* e) {
     fill(e->Kz, e->kz, (double)e->n);
 }
 
-// kinetic factors stay FP64 in both precisions
+// Kinetic factors in FP64; for complex64 also K_x and the products K_y K_z
+// per (iy, iz), each as a float pair hi + lo with hi + lo = K to ~1e-15.
 int upload_kinetic(gpe_engine_s* e) {
     std::vector<cplx<double>> K;
     for (auto& v : e->Kx) K.push_back(mk(v.real(), v.imag()));
     for (auto& v : e->Ky) K.push_back(mk(v.real(), v.imag()));
     for (auto& v : e->Kz) K.push_back(mk(v.real(), v.imag()));
+    HIP_TRY(hipStreamSynchronize(e->stream));
     HIP_TRY(hipMemcpy(e->Ktab, K.data(), K.size() * sizeof(cplx<double>), hipMemcpyHostToDevice));
+    if (e->Khl) {
+        std::vector<cplx<float>> h;
+        h.reserve(2 * ((size_t)e->nx + (size_t)e->ny * e->nz));
+        auto split = [&](std::complex<double> v) {
+            const float hr = (float)v.real(), hi = (float)v.imag();
+            h.push_back(mk(hr, hi));
+            h.push_back(mk((float)(v.real() - (double)hr), (float)(v.imag() - (double)hi)));
+        };
+        for (auto& v : e->Kx) split(v);
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) split(e->Ky[iy] * e->Kz[iz]);
+        HIP_TRY(hipMemcpy(e->Khl, h.data(), h.size() * sizeof(cplx<float>), hipMemcpyHostToDevice));
+    }
     return 0;
 }
 
@@ -229,6 +245,10 @@ PassArgs<T> base_args(gpe_engine_s* e) {
     a.kx = k;
     a.ky = k + e->nx;
     a.kz = k + e->nx + e->ny;
+    if constexpr (sizeof(T) == 4) {
+        a.kx_hl = (const cplx<float>*)e->Khl;
+        a.kyz_hl = e->Khl ? (const cplx<float>*)e->Khl + 2 * e->nx : nullptr;
+    }
     a.kx64 = e->k64;
     a.ky64 = e->k64 + e->nx;
     a.kz64 = e->k64 + e->nx + e->ny;
@@ -468,7 +488,7 @@ int get_psi_t(gpe_engine_s* e, cuCplx* psi) {
 
 void free_all(gpe_engine_s* e) {
     void* ptrs[] = {e->A, e->B, e->C, e->Vb, e->Vq, e->varr, e->tw[0], e->tw[1], e->tw[2], e->twlo[0], e->twlo[1],
-                    e->twlo[2], e->Ktab, e->ktab, e->vtab,
+                    e->twlo[2], e->Ktab, e->ktab, e->Khl, e->vtab,
                     e->k64, e->partials, e->scal, e->rout, e->phase, e->wind};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -534,6 +554,8 @@ int create_impl(gpe_engine_s** out, const gpe_config_t* cfg, double alpha, doubl
     }
     ALLOC(e->Ktab, (size_t)(e->nx + e->ny + e->nz) * sizeof(cplx<double>));
     ALLOC(e->ktab, (size_t)(e->nx + e->ny + e->nz) * es);
+    if (e->precision == GPE_PRECISION_C64)
+        ALLOC(e->Khl, 2 * ((size_t)e->nx + (size_t)e->ny * e->nz) * sizeof(cplx<float>));
     ALLOC(e->vtab, (size_t)(e->nx + e->ny + e->nz) * es);
     ALLOC(e->k64, (size_t)(e->nx + e->ny + e->nz) * sizeof(double));
     // partial sums: the largest grid of any pass times three sums

@@ -337,6 +337,8 @@ struct PassArgs {
     const T* kx;           // wave numbers
     const T* ky;
     const T* kz;
+    const cplx<float>* kx_hl;   // complex64: K_x as (hi, lo) float pairs
+    const cplx<float>* kyz_hl;  // complex64: K_y K_z per (iy, iz) as (hi, lo) pairs
     const double* kx64;    // wave numbers in FP64 for the energy sums
     const double* ky64;
     const double* kz64;
@@ -355,23 +357,41 @@ struct Coord {
 };
 
 
-// v * exp(re + i im).  In complex64 the factor's angle and modulus come from
-// single-precision sincos/exp, the direction is renormalised to exact unit
-// modulus and the product taken in FP64, so the rounding of the factor does
-// not bias |psi| the same way every step.
+// sin and cos of x.  For |x| <= 1/4 (every real-time potential phase
+// c V = -dt V / 2 of a sane step) the Taylor polynomials to x^7 and x^6, whose
+// truncation error is below 4e-10; otherwise the library sincosf.
+__device__ __forceinline__ void sincos_small(float x, float* s, float* c) {
+    if (fabsf(x) <= 0.25f) {
+        const float t = x * x;
+        *s = fmaf(x * t, fmaf(t, fmaf(t, -1.0f / 5040.0f, 1.0f / 120.0f), -1.0f / 6.0f), x);
+        *c = fmaf(t, fmaf(t, fmaf(t, -1.0f / 720.0f, 1.0f / 24.0f), -0.5f), 1.0f);
+    } else {
+        sincosf(x, s, c);
+    }
+}
+
+// v * exp(re + i im).  In complex64 the rounded direction (c, s) misses unit
+// modulus by e = c^2 + s^2 - 1 ~ 1e-8, the same way every step; e is formed
+// without cancellation by two FMAs and the product is scaled by 1 - e/2
+// (= 1/|(c, s)| to O(e^2)), so the factor is a rotation to ~1e-15 and only
+// unbiased final roundings remain.
 __device__ __forceinline__ cplx<double> mul_exp(cplx<double> v, double re, double im) {
     return cmul(v, cexp_(re, im));
 }
 __device__ __forceinline__ cplx<float> mul_exp(cplx<float> v, float re, float im) {
     float s, c;
-    sincosf(im, &s, &c);
-    const double cd = (double)c, sd = (double)s;
-    // 1/|(c, s)| to O((m-1)^2) ~ 1e-13, |m - 1| being a few float ulps
-    const double m = cd * cd + sd * sd;
-    double f = 1.5 - 0.5 * m;
-    if (re != 0.0f) f *= (double)expf(re);  // real time: re == 0 for every point
-    const double vx = (double)v.x, vy = (double)v.y;
-    return mk((float)(f * (vx * cd - vy * sd)), (float)(f * (vx * sd + vy * cd)));
+    sincos_small(im, &s, &c);
+    const float h = -0.5f * fmaf(s, s, fmaf(c, c, -1.0f));
+    float rx = fmaf(v.x, c, -(v.y * s));
+    float ry = fmaf(v.x, s, v.y * c);
+    rx = fmaf(h, rx, rx);
+    ry = fmaf(h, ry, ry);
+    if (re != 0.0f) {  // real time: re == 0 for every point
+        const float g = expf(re);
+        rx *= g;
+        ry *= g;
+    }
+    return mk(rx, ry);
 }
 
 
@@ -397,6 +417,15 @@ template <typename T>
 __device__ __forceinline__ cplx<T> mid_op_lean(const PassArgs<T>& a, const Coord& c, cplx<T> v) {
     switch (a.mid_mode) {
     case MID_EXPT: {
+        if constexpr (sizeof(T) == 4) {
+            if (a.axis == 0 && a.kx_hl) {
+                // complex64 on the x axis: two compensated products with K_x and
+                // the per-pencil K_y K_z (hi + lo float pairs of FP64 values)
+                const unsigned yz = c.iy * (unsigned)a.ph.nz + c.iz;
+                v = cmul_comp(v, a.kx_hl[2 * c.ix], a.kx_hl[2 * c.ix + 1]);
+                return cmul_comp(v, a.kyz_hl[2 * yz], a.kyz_hl[2 * yz + 1]);
+            }
+        }
         cplx<double> w = mk((double)v.x, (double)v.y);
         w = cmul(w, a.Kx[c.ix]);
         w = cmul(w, a.Ky[c.iy]);
@@ -435,18 +464,17 @@ __device__ __forceinline__ T pot2(const PassArgs<T>& a, T vx, cplx<T> v, bool qf
     return lv;
 }
 
-template <typename T>
-__device__ __forceinline__ cplx<T> load_op2(const PassArgs<T>& a, const ElemIn<T>& e, cplx<T> v, double* acc) {
+template <int LM, typename T>
+__device__ __forceinline__ cplx<T> load_op_m(const PassArgs<T>& a, const ElemIn<T>& e, cplx<T> v, double* acc) {
     const bool qf = (a.flags & F_QF) != 0;
-    if (a.load_mode == LOAD_PLAIN) {
+    if constexpr (LM == LOAD_PLAIN) {
         if (a.flags & F_RED_POT) {
             const double rho = dens64((double)a.ph.gamma, (double)v.x, (double)v.y);
             acc[0] += rho * (double)e.vx;
             acc[1] += edf(a.ph.interactions, a.up64, rho);
         }
         return v;
-    }
-    if (a.load_mode == LOAD_PSI0) {
+    } else if constexpr (LM == LOAD_PSI0) {
         // v is psi0 (orginal/gpe_engine.cu:723-772 after the rescale / imprint)
         bool wb = false;
         if (a.flags & F_SCALE) { v = cscale(v, (T)a.scal[1]); wb = true; }
@@ -457,25 +485,32 @@ __device__ __forceinline__ cplx<T> load_op2(const PassArgs<T>& a, const ElemIn<T
             v = mul_exp(v, a.ph.c.x * lv, a.ph.c.y * lv);
         }
         return v;
+    } else {
+        // LOAD_PART2: v is the predictor phi3 (orginal/gpe_engine.cu:817-868)
+        cplx<T> phi3 = v;
+        if (a.flags & F_SCALE) phi3 = cscale(phi3, (T)a.scal[1]);
+        cplx<T> psi0 = e.aux;
+        const T V0 = pot2(a, e.vx, psi0, qf, e.raux);
+        T lv;
+        if (qf) lv = (T)0.5 * (V0 + e.vx + dedfdn(a.ph, density(a.ph, phi3)) + e.raux);
+        else lv = (T)0.5 * (V0 + e.vx + dedfdn(a.ph, density(a.ph, phi3)));
+        a.vbuf[e.c.off] = lv;
+        if (a.flags & F_IMPRINT_MID) psi0 = apply_pin(a, psi0, e.c);
+        return mul_exp(psi0, a.ph.c.x * lv, a.ph.c.y * lv);
     }
-    // LOAD_PART2: v is the predictor phi3 (orginal/gpe_engine.cu:817-868)
-    cplx<T> phi3 = v;
-    if (a.flags & F_SCALE) phi3 = cscale(phi3, (T)a.scal[1]);
-    cplx<T> psi0 = e.aux;
-    const T V0 = pot2(a, e.vx, psi0, qf, e.raux);
-    T lv;
-    if (qf) lv = (T)0.5 * (V0 + e.vx + dedfdn(a.ph, density(a.ph, phi3)) + e.raux);
-    else lv = (T)0.5 * (V0 + e.vx + dedfdn(a.ph, density(a.ph, phi3)));
-    a.vbuf[e.c.off] = lv;
-    if (a.flags & F_IMPRINT_MID) psi0 = apply_pin(a, psi0, e.c);
-    return mul_exp(psi0, a.ph.c.x * lv, a.ph.c.y * lv);
 }
 
 template <typename T>
-__device__ __forceinline__ cplx<T> mid_op2(const PassArgs<T>& a, const ElemIn<T>& e, cplx<T> v, double* acc) {
+__device__ __forceinline__ cplx<T> load_op2(const PassArgs<T>& a, const ElemIn<T>& e, cplx<T> v, double* acc) {
+    if (a.load_mode == LOAD_PLAIN) return load_op_m<LOAD_PLAIN>(a, e, v, acc);
+    if (a.load_mode == LOAD_PSI0) return load_op_m<LOAD_PSI0>(a, e, v, acc);
+    return load_op_m<LOAD_PART2>(a, e, v, acc);
+}
+
+template <int MM, typename T>
+__device__ __forceinline__ cplx<T> mid_op_m(const PassArgs<T>& a, const ElemIn<T>& e, cplx<T> v, double* acc) {
     const bool qf = (a.flags & F_QF) != 0;
-    switch (a.mid_mode) {
-    case MID_VSTEP2: {
+    if constexpr (MM == MID_VSTEP2) {
         // orginal/gpe_engine.cu:774-801 (and the _part1/_part2[_qf] pair for qf)
         const cplx<T> psi0 = e.aux;
         const T V0 = pot2(a, e.vx, psi0, qf, e.raux);
@@ -485,15 +520,13 @@ __device__ __forceinline__ cplx<T> mid_op2(const PassArgs<T>& a, const ElemIn<T>
         else lv = (T)0.5 * (V0 + e.vx + dedfdn(a.ph, density(a.ph, phi3)));
         a.vbuf[e.c.off] = lv;
         return mul_exp(psi0, a.ph.c.x * lv, a.ph.c.y * lv);
-    }
-    case MID_PART1: {
+    } else if constexpr (MM == MID_PART1) {
         // orginal/gpe_engine.cu:803-815 with the density sum of gpe_normalize fused
         const T V0 = pot2(a, e.vx, e.aux, qf, e.raux);
         const cplx<T> phi3 = mul_exp(v, a.ph.c.x * V0, a.ph.c.y * V0);
         acc[0] += dens64((double)a.ph.gamma, (double)phi3.x, (double)phi3.y);
         return phi3;
-    }
-    case MID_VSTEP3: {
+    } else if constexpr (MM == MID_VSTEP3) {
         // orginal/gpe_engine.cu:870-878; the factor exp(c V1) is recomputed from V1
         const T lv = e.raux;
         cplx<T> psi1 = mul_exp(v, a.ph.c.x * lv, a.ph.c.y * lv);
@@ -509,8 +542,7 @@ __device__ __forceinline__ cplx<T> mid_op2(const PassArgs<T>& a, const ElemIn<T>
         }
         a.psiA[e.c.off] = psi1;
         return psi1;
-    }
-    case MID_QF_VSTEP1: {
+    } else if constexpr (MM == MID_QF_VSTEP1) {
         // V_qf = Im(conj psi0 * F^-1[qf' k^2 F psi0]) / (rho0 + eps) (orginal 911-922), then Vstep1_qf
         const cplx<T> psi0 = e.aux;
         const T r0 = density(a.ph, psi0);
@@ -518,24 +550,35 @@ __device__ __forceinline__ cplx<T> mid_op2(const PassArgs<T>& a, const ElemIn<T>
         a.vq[e.c.off] = q;
         const T lv = pot2(a, e.vx, psi0, true, q);
         return mul_exp(psi0, a.ph.c.x * lv, a.ph.c.y * lv);
-    }
-    case MID_KIN_SUM: {
+    } else if constexpr (MM == MID_KIN_SUM) {
         const double k2 = a.kx64[e.c.ix] * a.kx64[e.c.ix] + a.ky64[e.c.iy] * a.ky64[e.c.iy] +
                           a.kz64[e.c.iz] * a.kz64[e.c.iz];
         acc[0] += k2 * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
         return v;
-    }
-    case MID_OVERLAP_RE: {
+    } else if constexpr (MM == MID_OVERLAP_RE) {
         const cplx<T> p = e.aux;
         a.rout[e.c.off] = (double)(p.x * v.x + p.y * v.y);
         return v;
+    } else if constexpr (MM == MID_EXPT || MM == MID_K2 || MM == MID_KA) {
+        return mid_op_lean(a, e.c, v);
+    } else {
+        return v;
     }
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> mid_op2(const PassArgs<T>& a, const ElemIn<T>& e, cplx<T> v, double* acc) {
+    switch (a.mid_mode) {
+    case MID_VSTEP2: return mid_op_m<MID_VSTEP2>(a, e, v, acc);
+    case MID_PART1: return mid_op_m<MID_PART1>(a, e, v, acc);
+    case MID_VSTEP3: return mid_op_m<MID_VSTEP3>(a, e, v, acc);
+    case MID_QF_VSTEP1: return mid_op_m<MID_QF_VSTEP1>(a, e, v, acc);
+    case MID_KIN_SUM: return mid_op_m<MID_KIN_SUM>(a, e, v, acc);
+    case MID_OVERLAP_RE: return mid_op_m<MID_OVERLAP_RE>(a, e, v, acc);
     case MID_EXPT:
     case MID_K2:
-    case MID_KA:
-        return mid_op_lean(a, e.c, v);
-    default:
-        return v;
+    case MID_KA: return mid_op_lean(a, e.c, v);
+    default: return v;
     }
 }
 

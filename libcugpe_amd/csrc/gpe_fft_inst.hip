@@ -10,6 +10,10 @@ namespace gpe {
 template <typename T, int NN>
 unsigned launch_fft_pass_n_impl(const PassArgs<T>& a, hipStream_t s) {
     static_assert(plan_ok(NN), "bad FFT plan");
+    if (a.axis == 2) {
+        const unsigned zb = launch_zwave<T, NN>(a, s);
+        if (zb) return zb;
+    }
     // instantiated variants: z lean {F, I}, z full {F, I, IF}, y/x lean {F, I, FI}, y/x full {F}
     const bool lean = pass_is_lean(a);
     const int W = lean ? tile_w<T, NN>() : (a.axis == 2 ? full_tile_w<T, NN, true>() : full_tile_w<T, NN, false>());

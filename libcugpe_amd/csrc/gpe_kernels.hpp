@@ -3,6 +3,7 @@
 #pragma once
 
 #include "gpe_fft.hpp"
+#include "gpe_zwave.hpp"
 
 namespace gpe {
 

@@ -18,10 +18,17 @@ unsigned launch_fft_pass_impl(int N, const PassArgs<T>& a, hipStream_t s) {
 #undef GPE_CASE
 }
 
+template <typename T, int NN>
+constexpr int min_tile() {
+    constexpr int w = full_tile_w<T, NN, true>();
+    if constexpr (zw_lanes(NN) != 0) return ZW<T, NN>::PENCILS < w ? ZW<T, NN>::PENCILS : w;
+    return w;
+}
+
 template <typename T>
 int fft_tile_width_impl(int N) {
 #define GPE_CASE(NN) \
-    case NN: return full_tile_w<T, NN, true>();  /* the narrowest tile of any pass */
+    case NN: return min_tile<T, NN>();  /* the narrowest tile of any pass */
     switch (N) {
         GPE_FFT_LENGTHS(GPE_CASE)
     default:

@@ -240,3 +240,33 @@ def test_c64_energy_error_vs_fp64(oracle, state, steps, tol):
     do, _ = o.get_density()
     assert np.max(np.abs(dg - do)) / np.max(do) < 1e-3
     g.close()
+
+
+@pytest.mark.parametrize("variant", ["vortex2", "vortex3", "phase"])
+@pytest.mark.parametrize("mode", ["rte", "ite", "ite_qf"])
+def test_pinned_variants_match_oracle(oracle, variant, mode):
+    say("pinned", variant, mode)
+    alpha, beta = (1.0, 0.0) if mode == "rte" else (0.0, 1.0)
+    shape = (32, 24, 16)
+    g, o = pair(oracle, shape, alpha, beta, 0.025, 700.0, 0, 0, [0.05, 0.06, 0.1])
+    psi = seeded_state(shape, 19)
+    psi[3, 5, :] = 0.0  # exact zeros exercise the guarded imprint
+    rng = np.random.default_rng(4)
+    target = rng.uniform(-np.pi, np.pi, shape)
+    for x in (g, o):
+        if mode == "ite_qf":
+            x.set_quantum_friction_coeff(0.2)
+        x.set_psi(psi)
+        x.normalize_psi()
+        x.set_vortex(0.5, -2.5, 1)
+        if variant == "phase":
+            x.set_phase(target)
+    fn = {"vortex2": "evolve_vortex2", "vortex3": "evolve_vortex3", "phase": "evolve_enforced_phase"}[variant]
+    mg = getattr(g, fn)(6, want_mu=True)
+    mo = getattr(o, fn)(6, want_mu=True)
+    assert rel_err(g.get_psi()[0], o.get_psi()[0]) < TOL128
+    if mode != "rte":
+        assert abs(mg - mo) < TOL128 * abs(mo)
+    # gpe_get_phase: arg psi, kept as the enforced phase
+    assert np.max(np.abs(g.get_phase() - o.get_phase())) < 1e-10
+    g.close()
