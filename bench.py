@@ -102,6 +102,108 @@ def cpu_baseline(shape, psi, steps):
     }
 
 
+ENS_MIX = ((24, 256), (32, 128), (48, 64), (64, 64))  # 512 lattices, sizes 24^3:32^3:48^3:64^3 = 4:2:1:1
+ENS_PARAMS = [0.1, 4.0, 0.0, 0.008]                    # cosine lattice A=0.1, P=4; contact a=0.008
+
+
+def ground_state(E, L, ite_steps=300):
+    """Imaginary-time ground state of one L^3 lattice (one particle per site)."""
+    shape = (L, L, L)
+    n = L ** 3
+    g = E.Engine(shape, 0.0, 1.0, DT, float(n), E.CONTACT, E.COSINE, 0, E.C128)
+    g.set_user_params(ENS_PARAMS)
+    g.set_psi(np.ones(shape, dtype=np.complex128))
+    g.normalize_psi()
+    g.evolve_qf(ite_steps)
+    psi, _ = g.get_psi()
+    g.close()
+    return psi
+
+
+def wigner_sample(psi0, seed):
+    """psi0 plus complex Gaussian noise of variance 1/2 per plane-wave mode."""
+    rng = np.random.default_rng(seed)
+    shape = psi0.shape
+    beta = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) * 0.5
+    return psi0 + np.fft.ifftn(beta) * np.sqrt(psi0.size)
+
+
+def ensemble_bench(E, steps, seq_steps, ref_members, ref_steps):
+    from time import perf_counter
+
+    shapes, seeds = [], []
+    gs = {L: ground_state(E, L) for L, _ in ENS_MIX}
+    for L, cnt in ENS_MIX:
+        for r in range(cnt):
+            shapes.append((L, L, L))
+            seeds.append(5000 + len(seeds))
+    ens = E.Ensemble(shapes, 1.0, 0.0, DT, 1.0, E.CONTACT, E.COSINE, 0, E.C64)
+    ens.set_user_params(ENS_PARAMS)
+    for i, (sh, sd) in enumerate(zip(shapes, seeds)):
+        ens.set_psi(i, wigner_sample(gs[sh[0]], sd))
+    ens.autotune(3)
+    ens.evolve(5)  # warm-up (builds the step graphs)
+    t0 = perf_counter()
+    ens.evolve(steps)
+    t_ens = perf_counter() - t0
+    points = sum(int(np.prod(s)) for s in shapes)
+
+    # sequential per-lattice launches: one engine per shape, each lattice run on its own
+    seq_step_s = 0.0
+    for L, cnt in ENS_MIX:
+        g = E.Engine((L, L, L), 1.0, 0.0, DT, float(L ** 3), E.CONTACT, E.COSINE, 0, E.C64)
+        g.set_user_params(ENS_PARAMS)
+        g.set_psi(wigner_sample(gs[L], 1))
+        g.evolve_qf(5)
+        g.synchronize()
+        t0 = perf_counter()
+        g.evolve_qf(seq_steps)
+        g.synchronize()
+        seq_step_s += cnt * (perf_counter() - t0) / seq_steps
+        g.close()
+
+    # mean density of the first lattices of the smallest shape against the FP64 oracle
+    from oracle import pyoracle
+
+    L0 = ENS_MIX[0][0]
+    members = list(range(ref_members))
+    ens_ref = E.Ensemble([(L0,) * 3] * ref_members, 1.0, 0.0, DT, 1.0, E.CONTACT, E.COSINE, 0, E.C64)
+    ens_ref.set_user_params(ENS_PARAMS)
+    rho_o = np.zeros((L0,) * 3)
+    for i in members:
+        psi = wigner_sample(gs[L0], seeds[i])
+        ens_ref.set_psi(i, psi)
+        o = pyoracle.OracleEngine((L0,) * 3, 1.0, 0.0, DT, float(L0 ** 3), -1, 2, 0)
+        o.set_user_params(ENS_PARAMS)
+        o.set_psi(psi)
+        o.evolve_qf(ref_steps)
+        rho_o += np.abs(o.get_psi()[0]) ** 2 / ref_members
+        o.close()
+    ens_ref.evolve(ref_steps)
+    rho_g = ens_ref.mean_density(0)
+    ens_ref.close()
+    groups = ens.groups()
+    ens.close()
+    sps = steps / t_ens
+    S22 = 22 * 8 * points
+    return {
+        "workload": "512 truncated-Wigner lattices 24^3:32^3:48^3:64^3 = 256:128:64:64 (%d points), complex64 "
+        "real time, contact a=0.008 + cosine V_ext, ground state + variance-1/2 plane-wave noise" % points,
+        "ensemble_steps_per_s": sps,
+        "lattice_steps_per_s": sps * len(shapes),
+        "ms_per_ensemble_step": 1e3 / sps,
+        "sequential_ensemble_steps_per_s": 1.0 / seq_step_s,
+        "speedup_vs_sequential": sps * seq_step_s,
+        "sequential_method": "one engine per shape, %d steps each, per-step time x lattices of that shape" % seq_steps,
+        "hbm_frac": S22 * sps / (HBM_PEAK_GBS * 1e9),
+        "graph": True,
+        "groups": [{"shape": list(g[0]), "lattices": g[1], "variant": g[2]} for g in groups],
+        "mean_density_rel_err_vs_fp64": float(np.linalg.norm(rho_g - rho_o) / np.linalg.norm(rho_o)),
+        "mean_density_max_rel_err_vs_fp64": float(np.max(np.abs(rho_g - rho_o)) / np.max(rho_o)),
+        "mean_density_sample": "%d lattices of %d^3, %d steps, FP64 oracle" % (ref_members, L0, ref_steps),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +213,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c128", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this engine runs on one GPU; a %d-GPU run is not part of this build "
@@ -209,6 +313,10 @@ def main():
         result["c128_step_hbm_frac"] = 22 * 2 * S * (a.steps / t128) / (HBM_PEAK_GBS * 1e9)
         result["energy_rel_err_c64_vs_c128"] = abs(e64 - eref) / abs(eref)
         result["energy_steps_compared"] = total_steps
+
+    if not a.no_ensemble:
+        print("[bench] batched ensemble (%d steps)" % a.ensemble_steps, file=sys.stderr, flush=True)
+        result["ensemble"] = ensemble_bench(E, a.ensemble_steps, 50, 8, 100)
 
     if not a.no_cpu_baseline:
         print("[bench] FP64 CPU oracle baseline (%d steps)" % a.cpu_steps, file=sys.stderr, flush=True)

@@ -203,6 +203,55 @@ int gpe_engine_profile_read(gpe_engine_t e, double *total_ms, long *launches);
 /* library build information (GPU target, compiled FFT lengths) */
 const char *gpe_build_info(void);
 
+/* Launcher choice of an engine, picked by timing (no counterpart in the
+ * reference, whose transforms are cuFFT plans): bit 0 runs the z passes in
+ * LDS tiles instead of wave registers, bit 1 halves the tile width of the
+ * strided passes.  autotune times `steps` real-time steps per choice on the
+ * current state (restored afterwards) and keeps the fastest. */
+int gpe_engine_autotune(gpe_engine_t e, int steps, int *variant);
+int gpe_engine_set_variant(gpe_engine_t e, int variant);
+
+/* Adjoint of a real-time run (no counterpart in the reference).  forward:
+ * nt steps of gpe_evolve_qf from the current psi, keeping every ckpt-th state
+ * (0: ceil(sqrt(nt))), and *loss = 1 - |<psi_t|psi(T)>|^2 / (|psi_t|^2
+ * |psi(0)|^2).  backward: dL/dV_ext at every point (N doubles) by the
+ * discrete adjoint of the split step, recomputing one segment of states at a
+ * time; psi stays psi(T).  Real time without quantum friction, INTERACTIONS
+ * 0, -1 or -2 (GPE_ERR_ARG otherwise); V_ext any VEXT (gradient with respect
+ * to the per-point values the step uses). */
+int gpe_engine_adjoint_forward(gpe_engine_t e, int nt, const cuCplx *psi_target, int ckpt, double *loss);
+int gpe_engine_adjoint_backward(gpe_engine_t e, double *grad_vext);
+
+/* ------------------------------------------------------------------------
+ * Ensembles: many independent lattices of possibly different shapes
+ * (truncated-Wigner realisations, parameter sweeps) advanced together in
+ * real time.  No counterpart in the reference, which holds one lattice per
+ * process (dev/gpe_engine.cu:43-44); every lattice follows the same split
+ * step as gpe_evolve_qf.  Lattices of one shape share one batched engine;
+ * a step is one HIP graph launch covering every group.
+ * --------------------------------------------------------------------- */
+
+typedef struct gpe_ensemble_s *gpe_ensemble_t;
+
+/* dims: ninst triples (nx, ny, nz); alpha/beta/dt/npart as gpe_create_engine.
+ * Evolution is real time only (beta must be 0 when gpe_ensemble_evolve runs). */
+int gpe_ensemble_create(gpe_ensemble_t *out, int ninst, const int *dims, int interactions, int vext, int gpe_for,
+                        int precision, double alpha, double beta, double dt, double npart);
+int gpe_ensemble_destroy(gpe_ensemble_t s);
+int gpe_ensemble_set_user_params(gpe_ensemble_t s, int size, const double *params);
+int gpe_ensemble_set_quantum_friction_coeff(gpe_ensemble_t s, double qfcoeff);
+int gpe_ensemble_set_psi(gpe_ensemble_t s, int inst, const cuCplx *psi);
+int gpe_ensemble_get_psi(gpe_ensemble_t s, int inst, double *t, cuCplx *psi);
+int gpe_ensemble_evolve(gpe_ensemble_t s, int nt);
+/* 1: replay captured HIP graphs (default), 0: launch the passes directly */
+int gpe_ensemble_set_graph(gpe_ensemble_t s, int on);
+int gpe_ensemble_autotune(gpe_ensemble_t s, int steps);
+int gpe_ensemble_groups(gpe_ensemble_t s, int *ngroups);
+int gpe_ensemble_group_info(gpe_ensemble_t s, int group, int *nx, int *ny, int *nz, int *count, int *variant);
+int gpe_ensemble_instance(gpe_ensemble_t s, int inst, int *group, int *slot);
+/* mean of gamma |psi|^2 over the lattices of one group (FP64, nx*ny*nz values) */
+int gpe_ensemble_mean_density(gpe_ensemble_t s, int group, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,58 @@
+"""PyTorch face of the engine's adjoint (gpe_engine_adjoint_forward/_backward).
+
+`OverlapLoss` is a torch.autograd.Function: its forward sets V_ext on an
+engine, runs nt real-time steps from psi0 and returns
+L = 1 - |<psi_t|psi(T)>|^2 / (|psi_t|^2 |psi(0)|^2); its backward runs the
+hand-written discrete adjoint of the split step on the GPU (sqrt(nt)
+checkpoints) and returns dL/dV_ext.  `gaussian_bumps` builds a potential from
+bump amplitudes with torch ops, so autograd carries dL/dV_ext on to the
+amplitudes.  The engine works on host arrays at its C ABI; tensors are moved
+to and from the CPU at this boundary.
+"""
+import numpy as np
+import torch
+
+from . import engine as E
+
+
+class OverlapLoss(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, V, eng, psi0, psi_target, nt, ckpt=0):
+        v = V.detach().to("cpu", torch.float64).contiguous().numpy()
+        eng.set_external_potential(v)
+        eng.set_psi(psi0)
+        loss = eng.adjoint_forward(nt, psi_target, ckpt)
+        ctx.eng = eng
+        return torch.tensor(loss, dtype=V.dtype, device=V.device)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        g = torch.from_numpy(ctx.eng.adjoint_backward()).to(device=grad_out.device, dtype=grad_out.dtype)
+        return grad_out * g, None, None, None, None, None
+
+
+def overlap_loss(V, eng, psi0, psi_target, nt, ckpt=0):
+    return OverlapLoss.apply(V, eng, psi0, psi_target, nt, ckpt)
+
+
+def bump_centres(shape, count, seed):
+    """Seeded centres (lattice coordinates) of `count` bumps."""
+    rng = np.random.default_rng(seed)
+    return np.stack([rng.uniform(0, n, count) for n in shape], axis=1)
+
+
+def gaussian_bumps(amp, centres, width, shape):
+    """V(x) = sum_j amp_j exp(-|x - x_j|^2 / (2 width^2)) on the periodic
+    lattice (minimum-image distance per axis), separable per bump."""
+    dev, dt = amp.device, amp.dtype
+    factors = []
+    for a, n in enumerate(shape):
+        x = torch.arange(n, device=dev, dtype=dt)
+        c = torch.as_tensor(centres[:, a], device=dev, dtype=dt)
+        d = x[None, :] - c[:, None]
+        d = d - n * torch.round(d / n)
+        factors.append(torch.exp(-d * d / (2.0 * width * width)))
+    gx, gy, gz = factors
+    # sum_j a_j gx[j, ix] gy[j, iy] gz[j, iz] as two matrix products
+    t = (amp[:, None] * gx).T @ (gy[:, :, None] * gz[:, None, :]).reshape(len(amp), -1)
+    return t.reshape(shape)

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <cstdlib>
@@ -45,7 +46,9 @@ struct EventPair {
 
 struct gpe_engine_s {
     int nx = 0, ny = 0, nz = 0;
-    size_t n = 0;
+    size_t n = 0;        // points of one lattice
+    size_t batch = 1;    // lattices of this shape stored one after another (ensembles)
+    int variant = 0;     // kernel choice of the pass launcher (autotuned)
     int precision = GPE_PRECISION_C128;
     int interactions = 0, vext = 0, gpe_for = 0;
     double gamma = 2.0;
@@ -76,6 +79,16 @@ struct gpe_engine_s {
     double *rout = nullptr;  // FP64 real output (density / currents / phase)
     double *phase = nullptr; // enforced phase
     int8_t *wind = nullptr;
+
+    // adjoint of a real-time run (gpe_engine_adjoint_forward / _backward)
+    void *adj_ck = nullptr, *adj_seg = nullptr;  // checkpoints, one segment of states
+    size_t adj_ck_len = 0, adj_seg_len = 0;     // in states
+    void *adj_lam = nullptr, *adj_phi = nullptr, *adj_lam0 = nullptr, *adj_tgt = nullptr, *adj_fin = nullptr;
+    double *adj_vb = nullptr, *adj_grad = nullptr, *adj_vx = nullptr;
+    int adj_nt = 0, adj_C = 0;
+    unsigned adj_it0 = 0;
+    bool adj_ready = false;
+    double adj_fr = 0, adj_fi = 0;  // lambda(T) = (fr + i fi) psi_target
 
     // profiling
     int prof_class = -1;
@@ -286,7 +299,8 @@ template <typename T>
 int run_pass(gpe_engine_s* e, PassArgs<T> a, const PassSpec& s) {
     a.axis = s.axis;
     const int N = axis_len(e, s.axis);
-    a.M = (unsigned)(e->n / (size_t)N);
+    a.M = (unsigned)(e->n * e->batch / (size_t)N);
+    a.variant = e->variant;
     a.n1 = (unsigned)e->n;
     a.src = (const cplx<T>*)s.src;
     a.dst = (cplx<T>*)s.dst;
@@ -326,74 +340,103 @@ int normalize_t(gpe_engine_s* e) {
     return 0;
 }
 
-// nt split steps (orginal/gpe_engine.cu:942-1016, dev/gpe_engine.cu:1030-1202)
+// Step issue.  A run of nt split steps (orginal/gpe_engine.cu:942-1016,
+// dev/gpe_engine.cu:1030-1202) is the open pass followed by nt steps; every
+// step but the last ends in a pass that also opens the next one.
 // pin: 0 none, 1 vortex imprint, 2 imprint where |psi| > 1e-15, 3 stored phase;
 // mid_pin: re-applied after the predictor normalisation in imaginary time
+struct StepPlan {
+    int pin = 0;
+    bool mid_pin = false;
+};
+
 template <typename T>
-int evolve_t(gpe_engine_s* e, int nt, double* mu, int pin, bool mid_pin) {
-    if (nt <= 0) return 0;
+PassArgs<T> step_args(gpe_engine_s* e, const StepPlan& sp) {
+    PassArgs<T> a = base_args<T>(e);
+    a.kfac = (T)e->qf;
+    a.pin = sp.pin;
+    a.phase = e->phase;
+    return a;
+}
+
+template <typename T>
+void issue_open(gpe_engine_s* e, const StepPlan& sp) {
+    const bool qf = (e->qf != 0.0);
+    const int imp = sp.pin ? F_IMPRINT : 0;
+    PassArgs<T> a = step_args<T>(e, sp);
+    if (!qf)
+        run_pass<T>(e, a, {2, nullptr, e->B, -1, 0, LOAD_PSI0, MID_NONE, STORE_PLAIN, F_VSTEP1 | imp, 0});
+    else
+        run_pass<T>(e, a, {2, nullptr, e->C, -1, 0, LOAD_PSI0, MID_NONE, STORE_PLAIN, imp, 0});
+}
+
+// one split step from time index e->it (the host counter is advanced by the caller)
+template <typename T>
+void issue_step(gpe_engine_s* e, const StepPlan& sp, bool last) {
     const bool ite = (e->beta != 0.0);
     const bool qf = (e->qf != 0.0);
-    const bool vortex = (pin != 0);
+    const bool vortex = (sp.pin != 0);
     const int imp = vortex ? F_IMPRINT : 0;
     const int fqf = qf ? F_QF : 0;
     void *A = e->A, *B = e->B, *C = e->C;
     const int F = -1, I = +1;
+    PassArgs<T> a = step_args<T>(e, sp);
+    if (qf) {
+        run_pass<T>(e, a, {1, C, C, F, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+        run_pass<T>(e, a, {0, C, C, F, I, LOAD_PLAIN, MID_K2, STORE_PLAIN, 0, 0});
+        run_pass<T>(e, a, {1, C, C, I, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+        run_pass<T>(e, a, {2, C, B, I, F, LOAD_PLAIN, MID_QF_VSTEP1, STORE_PLAIN, F_QF, 0});
+    }
+    // predictor kinetic step of phi1
+    run_pass<T>(e, a, {1, B, B, F, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+    run_pass<T>(e, a, {0, B, B, F, I, LOAD_PLAIN, MID_EXPT, STORE_PLAIN, 0, 0});
+    run_pass<T>(e, a, {1, B, B, I, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+    if (!ite) {
+        run_pass<T>(e, a, {2, B, A, I, F, LOAD_PLAIN, MID_VSTEP2, STORE_PLAIN, fqf, 0});
+    } else {
+        const int nb = run_pass<T>(e, a, {2, B, B, I, 0, LOAD_PLAIN, MID_PART1, STORE_PLAIN, fqf, 1});
+        finalize(e, nb, 1, FIN_NORM, 0, 1.0);
+        run_pass<T>(e, a, {2, B, A, F, 0, LOAD_PART2, MID_NONE, STORE_PLAIN,
+                           F_SCALE | fqf | (sp.mid_pin ? F_IMPRINT_MID : 0), 0});
+    }
+    // corrector kinetic step of modify(psi0) W
+    run_pass<T>(e, a, {1, A, A, F, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+    run_pass<T>(e, a, {0, A, A, F, I, LOAD_PLAIN, MID_EXPT, STORE_PLAIN, 0, 0});
+    run_pass<T>(e, a, {1, A, A, I, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+    if (!ite) {
+        if (!last) {
+            const int nxt = (qf ? F_NEXT_PSI : F_NEXT_VSTEP1) | (vortex ? F_IMPRINT_NEXT : 0);
+            run_pass<T>(e, a, {2, A, qf ? C : B, I, F, LOAD_PLAIN, MID_VSTEP3, STORE_PLAIN, nxt, 0});
+        } else {
+            run_pass<T>(e, a, {2, A, nullptr, I, 0, LOAD_PLAIN, MID_VSTEP3, STORE_NONE, 0, 0});
+        }
+    } else {
+        const int nb = run_pass<T>(e, a, {2, A, nullptr, I, 0, LOAD_PLAIN, MID_VSTEP3, STORE_NONE, F_RED_NORM, 1});
+        finalize(e, nb, 1, FIN_NORM, 0, 1.0);
+        if (!last) {
+            if (!qf)
+                run_pass<T>(e, a, {2, nullptr, B, F, 0, LOAD_PSI0, MID_NONE, STORE_PLAIN, F_SCALE | F_VSTEP1 | imp, 0});
+            else
+                run_pass<T>(e, a, {2, nullptr, C, F, 0, LOAD_PSI0, MID_NONE, STORE_PLAIN, F_SCALE | imp, 0});
+        } else {
+            launch_scale<T>((cplx<T>*)A, (unsigned)e->n, e->scal, e->stream);
+        }
+    }
+}
 
-    PassArgs<T> a = base_args<T>(e);
-    a.kfac = (T)e->qf;
-    a.pin = pin;
-    a.phase = e->phase;
-    if (!qf)
-        run_pass<T>(e, a, {2, nullptr, B, F, 0, LOAD_PSI0, MID_NONE, STORE_PLAIN, F_VSTEP1 | imp, 0});
-    else
-        run_pass<T>(e, a, {2, nullptr, C, F, 0, LOAD_PSI0, MID_NONE, STORE_PLAIN, imp, 0});
-
+template <typename T>
+int evolve_t(gpe_engine_s* e, int nt, double* mu, int pin, bool mid_pin) {
+    if (nt <= 0) return 0;
+    const bool ite = (e->beta != 0.0);
+    // lattices of a batch share one stream of passes: imaginary time (one
+    // norm per lattice) and a stored phase (one array) are single-lattice only
+    if (e->batch > 1 && (ite || pin == 3)) return GPE_ERR_ARG;
+    StepPlan sp;
+    sp.pin = pin;
+    sp.mid_pin = mid_pin;
+    issue_open<T>(e, sp);
     for (int s = 0; s < nt; s++) {
-        const bool last = (s == nt - 1);
-        a.ph.it = e->it;
-        if (qf) {
-            run_pass<T>(e, a, {1, C, C, F, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
-            run_pass<T>(e, a, {0, C, C, F, I, LOAD_PLAIN, MID_K2, STORE_PLAIN, 0, 0});
-            run_pass<T>(e, a, {1, C, C, I, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
-            run_pass<T>(e, a, {2, C, B, I, F, LOAD_PLAIN, MID_QF_VSTEP1, STORE_PLAIN, F_QF, 0});
-        }
-        // predictor kinetic step of phi1
-        run_pass<T>(e, a, {1, B, B, F, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
-        run_pass<T>(e, a, {0, B, B, F, I, LOAD_PLAIN, MID_EXPT, STORE_PLAIN, 0, 0});
-        run_pass<T>(e, a, {1, B, B, I, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
-        if (!ite) {
-            run_pass<T>(e, a, {2, B, A, I, F, LOAD_PLAIN, MID_VSTEP2, STORE_PLAIN, fqf, 0});
-        } else {
-            const int nb = run_pass<T>(e, a, {2, B, B, I, 0, LOAD_PLAIN, MID_PART1, STORE_PLAIN, fqf, 1});
-            finalize(e, nb, 1, FIN_NORM, 0, 1.0);
-            run_pass<T>(e, a, {2, B, A, F, 0, LOAD_PART2, MID_NONE, STORE_PLAIN,
-                               F_SCALE | fqf | (mid_pin ? F_IMPRINT_MID : 0), 0});
-        }
-        // corrector kinetic step of modify(psi0) W
-        run_pass<T>(e, a, {1, A, A, F, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
-        run_pass<T>(e, a, {0, A, A, F, I, LOAD_PLAIN, MID_EXPT, STORE_PLAIN, 0, 0});
-        run_pass<T>(e, a, {1, A, A, I, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
-        if (!ite) {
-            if (!last) {
-                const int nxt = (qf ? F_NEXT_PSI : F_NEXT_VSTEP1) | (vortex ? F_IMPRINT_NEXT : 0);
-                run_pass<T>(e, a, {2, A, qf ? C : B, I, F, LOAD_PLAIN, MID_VSTEP3, STORE_PLAIN, nxt, 0});
-            } else {
-                run_pass<T>(e, a, {2, A, nullptr, I, 0, LOAD_PLAIN, MID_VSTEP3, STORE_NONE, 0, 0});
-            }
-        } else {
-            const int nb = run_pass<T>(e, a, {2, A, nullptr, I, 0, LOAD_PLAIN, MID_VSTEP3, STORE_NONE, F_RED_NORM, 1});
-            finalize(e, nb, 1, FIN_NORM, 0, 1.0);
-            if (!last) {
-                if (!qf)
-                    run_pass<T>(e, a, {2, nullptr, B, F, 0, LOAD_PSI0, MID_NONE, STORE_PLAIN,
-                                       F_SCALE | F_VSTEP1 | imp, 0});
-                else
-                    run_pass<T>(e, a, {2, nullptr, C, F, 0, LOAD_PSI0, MID_NONE, STORE_PLAIN, F_SCALE | imp, 0});
-            } else {
-                launch_scale<T>((cplx<T>*)A, (unsigned)e->n, e->scal, e->stream);
-            }
-        }
+        issue_step<T>(e, sp, s == nt - 1);
         e->it = e->it + 1;
     }
     HIP_TRY(hipGetLastError());
@@ -456,13 +499,14 @@ int currents_t(gpe_engine_s* e, double* js[3]) {
 }
 
 template <typename T>
-int set_psi_t(gpe_engine_s* e, const cuCplx* psi) {
+int set_psi_t(gpe_engine_s* e, const cuCplx* psi, size_t inst = 0) {
+    cplx<T>* dst = (cplx<T>*)e->A + inst * e->n;
     if (sizeof(T) == sizeof(double)) {
-        HIP_TRY(hipMemcpyAsync(e->A, psi, e->n * sizeof(cuCplx), hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(dst, psi, e->n * sizeof(cuCplx), hipMemcpyHostToDevice, e->stream));
     } else {
         std::vector<cplx<T>> tmp(e->n);
         for (size_t i = 0; i < e->n; i++) tmp[i] = mk((T)psi[i].x, (T)psi[i].y);
-        HIP_TRY(hipMemcpyAsync(e->A, tmp.data(), e->n * sizeof(cplx<T>), hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(dst, tmp.data(), e->n * sizeof(cplx<T>), hipMemcpyHostToDevice, e->stream));
         HIP_TRY(hipStreamSynchronize(e->stream));
     }
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -470,13 +514,14 @@ int set_psi_t(gpe_engine_s* e, const cuCplx* psi) {
 }
 
 template <typename T>
-int get_psi_t(gpe_engine_s* e, cuCplx* psi) {
+int get_psi_t(gpe_engine_s* e, cuCplx* psi, size_t inst = 0) {
+    const cplx<T>* src = (const cplx<T>*)e->A + inst * e->n;
     if (sizeof(T) == sizeof(double)) {
-        HIP_TRY(hipMemcpyAsync(psi, e->A, e->n * sizeof(cuCplx), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpyAsync(psi, src, e->n * sizeof(cuCplx), hipMemcpyDeviceToHost, e->stream));
         HIP_TRY(hipStreamSynchronize(e->stream));
     } else {
         std::vector<cplx<T>> tmp(e->n);
-        HIP_TRY(hipMemcpyAsync(tmp.data(), e->A, e->n * sizeof(cplx<T>), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpyAsync(tmp.data(), src, e->n * sizeof(cplx<T>), hipMemcpyDeviceToHost, e->stream));
         HIP_TRY(hipStreamSynchronize(e->stream));
         for (size_t i = 0; i < e->n; i++) {
             psi[i].x = (double)tmp[i].x;
@@ -489,7 +534,9 @@ int get_psi_t(gpe_engine_s* e, cuCplx* psi) {
 void free_all(gpe_engine_s* e) {
     void* ptrs[] = {e->A, e->B, e->C, e->Vb, e->Vq, e->varr, e->tw[0], e->tw[1], e->tw[2], e->twlo[0], e->twlo[1],
                     e->twlo[2], e->Ktab, e->ktab, e->Khl, e->vtab,
-                    e->k64, e->partials, e->scal, e->rout, e->phase, e->wind};
+                    e->k64, e->partials, e->scal, e->rout, e->phase, e->wind, e->adj_ck, e->adj_seg,
+                    e->adj_lam, e->adj_phi, e->adj_lam0, e->adj_tgt, e->adj_fin, e->adj_vb, e->adj_grad,
+                    e->adj_vx};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& ev : e->prof_ev) {
@@ -499,14 +546,15 @@ void free_all(gpe_engine_s* e) {
     if (e->stream) (void)hipStreamDestroy(e->stream);
 }
 
-int create_impl(gpe_engine_s** out, const gpe_config_t* cfg, double alpha, double beta, double dt, double npart) {
+int create_impl(gpe_engine_s** out, const gpe_config_t* cfg, double alpha, double beta, double dt, double npart,
+                size_t batch = 1) {
     if (!out || !cfg) return GPE_ERR_ARG;
     *out = nullptr;
     if (cfg->nx < 2 || cfg->ny < 2 || cfg->nz < 2) return GPE_ERR_LATTICE;
     if (!fft_length_supported(cfg->nx) || !fft_length_supported(cfg->ny) || !fft_length_supported(cfg->nz))
         return GPE_ERR_LATTICE;
     const size_t n = (size_t)cfg->nx * cfg->ny * cfg->nz;
-    if (n >= (size_t)1 << 31) return GPE_ERR_LATTICE;
+    if (batch < 1 || n * batch >= (size_t)1 << 31) return GPE_ERR_LATTICE;
     if (cfg->precision != GPE_PRECISION_C64 && cfg->precision != GPE_PRECISION_C128) return GPE_ERR_ARG;
     if (cfg->gpe_for != 0 && cfg->gpe_for != PARTICLES && cfg->gpe_for != DIMERS) return GPE_ERR_GAMMA;
     if (cfg->vext < 0 || cfg->vext > 3) return GPE_ERR_ARG;
@@ -517,6 +565,7 @@ int create_impl(gpe_engine_s** out, const gpe_config_t* cfg, double alpha, doubl
     e->ny = cfg->ny;
     e->nz = cfg->nz;
     e->n = n;
+    e->batch = batch;
     e->precision = cfg->precision;
     e->interactions = cfg->interactions;
     e->vext = cfg->vext;
@@ -545,9 +594,10 @@ int create_impl(gpe_engine_s** out, const gpe_config_t* cfg, double alpha, doubl
     err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (err != hipSuccess) return fail((int)err);
     const size_t es = esize(e);
-    ALLOC(e->A, n * 2 * es);
-    ALLOC(e->B, n * 2 * es);
-    ALLOC(e->Vb, n * es);
+    const size_t nb = n * batch;
+    ALLOC(e->A, nb * 2 * es);
+    ALLOC(e->B, nb * 2 * es);
+    ALLOC(e->Vb, nb * es);
     for (int a = 0; a < 3; a++) {
         ALLOC(e->tw[a], (size_t)axis_len(e, a) * 2 * es);
         if (e->precision == GPE_PRECISION_C64) ALLOC(e->twlo[a], (size_t)axis_len(e, a) * 2 * es);
@@ -563,13 +613,13 @@ int create_impl(gpe_engine_s** out, const gpe_config_t* cfg, double alpha, doubl
     for (int a = 0; a < 3; a++) {
         const int N = axis_len(e, a);
         const int W = (e->precision == GPE_PRECISION_C64) ? fft_tile_width<float>(N) : fft_tile_width<double>(N);
-        maxb = std::max(maxb, (n / N + W - 1) / W);
+        maxb = std::max(maxb, (nb / N + W - 1) / W);
     }
     e->partials_len = 3 * maxb;
     ALLOC(e->partials, e->partials_len * sizeof(double));
     ALLOC(e->scal, 16 * sizeof(double));
 #undef ALLOC
-    (void)hipMemset(e->A, 0, n * 2 * es);
+    (void)hipMemset(e->A, 0, nb * 2 * es);
     (void)hipMemset(e->scal, 0, 16 * sizeof(double));
     int r = (e->precision == GPE_PRECISION_C64) ? upload_static_tables<float>(e) : upload_static_tables<double>(e);
     if (!r) r = upload_potential_any(e);
@@ -653,8 +703,8 @@ int gpe_engine_set_quantum_friction_coeff(gpe_engine_t e, double qfcoeff) {
     if (qfcoeff != 0.0) {
         e->qf = qfcoeff / (e->gamma * (double)e->n);
         const size_t es = esize(e);
-        if (!e->C) HIP_TRY(hipMalloc(&e->C, e->n * 2 * es));
-        if (!e->Vq) HIP_TRY(hipMalloc(&e->Vq, e->n * es));
+        if (!e->C) HIP_TRY(hipMalloc(&e->C, e->n * e->batch * 2 * es));
+        if (!e->Vq) HIP_TRY(hipMalloc(&e->Vq, e->n * e->batch * es));
     } else {
         e->qf = 0.0;
         if (e->C) (void)hipFree(e->C);
@@ -679,12 +729,16 @@ int gpe_engine_set_external_potential(gpe_engine_t e, const double* v) {
     if (!e) return GPE_ERR_NO_ENGINE;
     if (!v) return GPE_ERR_ARG;
     const size_t es = esize(e);
-    if (!e->varr) HIP_TRY(hipMalloc(&e->varr, e->n * es));
-    if (es == 8) {
-        HIP_TRY(hipMemcpy(e->varr, v, e->n * sizeof(double), hipMemcpyHostToDevice));
-    } else {
-        std::vector<float> tmp(v, v + e->n);
-        HIP_TRY(hipMemcpy(e->varr, tmp.data(), e->n * sizeof(float), hipMemcpyHostToDevice));
+    // one copy per lattice of a batch: the kernels index the array by the global offset
+    if (!e->varr) HIP_TRY(hipMalloc(&e->varr, e->n * e->batch * es));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (size_t b = 0; b < e->batch; b++) {
+        if (es == 8) {
+            HIP_TRY(hipMemcpy((double*)e->varr + b * e->n, v, e->n * sizeof(double), hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> tmp(v, v + e->n);
+            HIP_TRY(hipMemcpy((float*)e->varr + b * e->n, tmp.data(), e->n * sizeof(float), hipMemcpyHostToDevice));
+        }
     }
     e->varr_set = true;
     return 0;
@@ -1022,6 +1076,508 @@ void gpe_print_potential_type(void) {
     static const char* names[] = {"HARMONIC POTENTIAL", "WOODS-SAXON TUBE", "COSINE LATTICE", "USER ARRAY"};
     int v = g_eng ? g_eng->vext : g_cfg.cfg.vext;
     printf("%s\n", (v >= 0 && v <= 3) ? names[v] : "UNKNOWN");
+}
+
+}  // extern "C"
+
+// ====================================================================== adjoint
+//
+// Gradient of L = 1 - |<psi_t|psi(T)>|^2 / (|psi_t|^2 |psi(0)|^2) with respect
+// to V_ext at every point for nt real-time steps (gpe_adjoint.hip holds the
+// pointwise parts and the derivation).  The forward run keeps the state every
+// C steps (C = ceil(sqrt(nt)) by default); the backward run recomputes one
+// segment of C states at a time from its checkpoint and walks it backwards:
+// memory C + nt/C states, work one extra forward run.
+
+namespace {
+
+int adj_alloc(void** p, size_t* have, size_t want, size_t unit) {
+    if (*have >= want && *p) return 0;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *have = 0;
+    HIP_TRY(hipMalloc(p, want * unit));
+    *have = want;
+    return 0;
+}
+
+// psi <- K psi (adjoint: conj K) on a buffer, in place
+template <typename T>
+void kinetic_apply(gpe_engine_s* e, void* buf, bool adjoint) {
+    PassArgs<T> a = base_args<T>(e);
+    run_pass<T>(e, a, {2, buf, buf, -1, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+    run_pass<T>(e, a, {1, buf, buf, -1, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+    run_pass<T>(e, a, {0, buf, buf, -1, 1, LOAD_PLAIN, adjoint ? MID_EXPTC : MID_EXPT, STORE_PLAIN, 0, 0});
+    run_pass<T>(e, a, {1, buf, buf, 1, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+    run_pass<T>(e, a, {2, buf, buf, 1, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+}
+
+template <typename T>
+AdjArgs adj_args(gpe_engine_s* e) {
+    AdjArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.interactions = e->interactions;
+    a.gamma = e->gamma;
+    for (int i = 0; i < kMaxUserParams; i++) a.up[i] = e->up[i];
+    a.cx = e->c.real();
+    a.cy = e->c.imag();
+    a.n = e->n;
+    a.vx = e->adj_vx;
+    a.phi = e->adj_phi;
+    a.lam = e->adj_lam;
+    a.lam0 = e->adj_lam0;
+    a.vb = e->adj_vb;
+    a.grad = e->adj_grad;
+    return a;
+}
+
+template <typename T>
+int adjoint_forward_t(gpe_engine_s* e, int nt, const cuCplx* tgt, int ckpt, double* loss) {
+    const size_t n = e->n, cs = sizeof(cplx<T>);
+    const int C = ckpt > 0 ? ckpt : (int)std::ceil(std::sqrt((double)nt));
+    const int nck = (nt + C - 1) / C;
+    size_t have;
+    e->adj_ready = false;
+    int r = adj_alloc(&e->adj_ck, &e->adj_ck_len, (size_t)nck * n, cs);
+    if (!r) r = adj_alloc(&e->adj_seg, &e->adj_seg_len, (size_t)(C + 1) * n, cs);
+    for (void** p : {&e->adj_lam, &e->adj_phi, &e->adj_lam0, &e->adj_tgt, &e->adj_fin}) {
+        have = *p ? n : 0;
+        if (!r) r = adj_alloc(p, &have, n, cs);
+    }
+    for (double** p : {&e->adj_vb, &e->adj_grad, &e->adj_vx}) {
+        have = *p ? n : 0;
+        if (!r) r = adj_alloc((void**)p, &have, n, sizeof(double));
+    }
+    if (r) return r;
+    // target in the working precision, V_ext as the forward kernels see it
+    {
+        std::vector<cplx<T>> t(n);
+        for (size_t i = 0; i < n; i++) t[i] = mk((T)tgt[i].x, (T)tgt[i].y);
+        HIP_TRY(hipMemcpyAsync(e->adj_tgt, t.data(), n * cs, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
+    launch_adjoint_vfill<T>(base_args<T>(e).ph, e->adj_vx, n, e->stream);
+    e->adj_it0 = e->it;
+    for (int k = 0; k < nt; k++) {
+        if (k % C == 0)
+            HIP_TRY(hipMemcpyAsync((char*)e->adj_ck + (size_t)(k / C) * n * cs, e->A, n * cs, hipMemcpyDeviceToDevice,
+                                   e->stream));
+        r = evolve_t<T>(e, 1, nullptr, 0, false);
+        if (r) return r;
+    }
+    const int nb = launch_adjoint_overlap<T>(e->A, e->adj_tgt, e->adj_ck, n, e->partials, e->stream);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h(4 * (size_t)nb);
+    HIP_TRY(hipMemcpyAsync(h.data(), e->partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    double s4[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 4; k++)
+        for (int b = 0; b < nb; b++) s4[k] += h[(size_t)k * nb + b];
+    const double sc = 1.0 / (s4[2] * s4[3]);
+    *loss = 1.0 - (s4[0] * s4[0] + s4[1] * s4[1]) * sc;
+    e->adj_fr = -sc * s4[0];
+    e->adj_fi = -sc * s4[1];
+    e->adj_nt = nt;
+    e->adj_C = C;
+    e->adj_ready = true;
+    return 0;
+}
+
+template <typename T>
+int adjoint_backward_t(gpe_engine_s* e, double* grad) {
+    const size_t n = e->n, cs = sizeof(cplx<T>);
+    const int nt = e->adj_nt, C = e->adj_C;
+    const int nck = (nt + C - 1) / C;
+    const unsigned it_end = e->it;
+    HIP_TRY(hipMemcpyAsync(e->adj_fin, e->A, n * cs, hipMemcpyDeviceToDevice, e->stream));
+    launch_adjoint_lam_init<T>(e->adj_lam, e->adj_tgt, e->adj_fr, e->adj_fi, n, e->stream);
+    HIP_TRY(hipMemsetAsync(e->adj_grad, 0, n * sizeof(double), e->stream));
+    AdjArgs a = adj_args<T>(e);
+    for (int sg = nck - 1; sg >= 0; sg--) {
+        const int k0 = sg * C, k1 = std::min(nt, k0 + C);
+        char* seg = (char*)e->adj_seg;
+        HIP_TRY(hipMemcpyAsync(e->A, (char*)e->adj_ck + (size_t)sg * n * cs, n * cs, hipMemcpyDeviceToDevice,
+                               e->stream));
+        HIP_TRY(hipMemcpyAsync(seg, e->A, n * cs, hipMemcpyDeviceToDevice, e->stream));
+        e->it = e->adj_it0 + (unsigned)k0;
+        for (int j = 1; j <= k1 - k0; j++) {
+            int r = evolve_t<T>(e, 1, nullptr, 0, false);
+            if (r) return r;
+            HIP_TRY(hipMemcpyAsync(seg + (size_t)j * n * cs, e->A, n * cs, hipMemcpyDeviceToDevice, e->stream));
+        }
+        for (int k = k1 - 1; k >= k0; k--) {
+            a.psi0 = seg + (size_t)(k - k0) * n * cs;
+            a.psi1 = seg + (size_t)(k - k0 + 1) * n * cs;
+            launch_adjoint_step<T>(a, 1, e->stream);
+            kinetic_apply<T>(e, e->adj_phi, false);
+            launch_adjoint_step<T>(a, 2, e->stream);
+            kinetic_apply<T>(e, e->adj_lam, true);
+            launch_adjoint_step<T>(a, 3, e->stream);
+            kinetic_apply<T>(e, e->adj_lam, true);
+            launch_adjoint_step<T>(a, 4, e->stream);
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemcpyAsync(e->A, e->adj_fin, n * cs, hipMemcpyDeviceToDevice, e->stream));
+    e->it = it_end;
+    HIP_TRY(hipMemcpyAsync(grad, e->adj_grad, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+bool adjoint_supported(const gpe_engine_s* e) {
+    return e->beta == 0.0 && e->qf == 0.0 && e->batch == 1 &&
+           (e->interactions == INT_UFG || e->interactions == INT_CONTACT || e->interactions == INT_DIPOLAR);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpe_engine_adjoint_forward(gpe_engine_t e, int nt, const cuCplx* psi_target, int ckpt, double* loss) {
+    if (!e) return GPE_ERR_NO_ENGINE;
+    if (nt < 1 || !psi_target || !loss || ckpt < 0 || !adjoint_supported(e)) return GPE_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return DISPATCH(e, adjoint_forward_t, e, nt, psi_target, ckpt, loss);
+}
+
+int gpe_engine_adjoint_backward(gpe_engine_t e, double* grad_vext) {
+    if (!e) return GPE_ERR_NO_ENGINE;
+    if (!grad_vext || !e->adj_ready || !adjoint_supported(e)) return GPE_ERR_ARG;
+    return DISPATCH(e, adjoint_backward_t, e, grad_vext);
+}
+
+}  // extern "C"
+
+// ===================================================================== ensembles
+//
+// An ensemble is a set of independent lattices (truncated-Wigner noise
+// realisations, parameter sweeps) advanced together in real time.  Lattices of
+// one shape form a group: one engine whose state arrays hold the group's
+// lattices one after another, so every fused pass of a step covers the whole
+// group in one launch (pencil addressing runs across lattice boundaries).  A
+// step of the ensemble issues each group's passes on the group's own stream,
+// forked from and joined back into the ensemble stream, and is captured once
+// as a HIP graph: open, middle and last step graphs, replayed nt - 2 times for
+// the middle.  Kernel arguments are frozen in the graphs, so the graphs are
+// rebuilt after every parameter change; the time index is one of those
+// arguments and is read by no built-in potential (all are time independent).
+
+struct gpe_ensemble_s {
+    std::vector<gpe_engine_s*> groups;
+    std::vector<int> inst_group;
+    std::vector<size_t> inst_slot;
+    int precision = GPE_PRECISION_C64;
+    hipStream_t main = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};  // open, middle step, last step
+    bool graphs_ok = false;
+    bool use_graph = true;
+};
+
+namespace {
+
+void ens_drop_graphs(gpe_ensemble_s* s) {
+    for (auto& g : s->graph) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    s->graphs_ok = false;
+}
+
+void ens_free(gpe_ensemble_s* s) {
+    ens_drop_graphs(s);
+    for (auto* g : s->groups) {
+        if (g->stream) (void)hipStreamSynchronize(g->stream);
+        free_all(g);
+        delete g;
+    }
+    for (auto ev : s->join) (void)hipEventDestroy(ev);
+    if (s->fork) (void)hipEventDestroy(s->fork);
+    if (s->main) (void)hipStreamDestroy(s->main);
+    delete s;
+}
+
+// what: 0 open pass, 1 a middle step, 2 the last step
+template <typename T>
+void ens_issue(gpe_ensemble_s* s, int what) {
+    (void)hipEventRecord(s->fork, s->main);
+    StepPlan sp;
+    for (size_t i = 0; i < s->groups.size(); i++) {
+        gpe_engine_s* g = s->groups[i];
+        (void)hipStreamWaitEvent(g->stream, s->fork, 0);
+        if (what == 0) issue_open<T>(g, sp);
+        else issue_step<T>(g, sp, what == 2);
+        (void)hipEventRecord(s->join[i], g->stream);
+        (void)hipStreamWaitEvent(s->main, s->join[i], 0);
+    }
+}
+
+template <typename T>
+int ens_build_graphs(gpe_ensemble_s* s) {
+    ens_drop_graphs(s);
+    for (int w = 0; w < 3; w++) {
+        HIP_TRY(hipStreamBeginCapture(s->main, hipStreamCaptureModeThreadLocal));
+        ens_issue<T>(s, w);
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamEndCapture(s->main, &g));
+        const hipError_t err = hipGraphInstantiate(&s->graph[w], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_TRY(err);
+    }
+    s->graphs_ok = true;
+    return 0;
+}
+
+template <typename T>
+int ens_evolve_t(gpe_ensemble_s* s, int nt) {
+    if (nt <= 0) return 0;
+    for (auto* g : s->groups)
+        if (g->beta != 0.0) return GPE_ERR_ARG;  // real time only: one norm per lattice is not kept
+    for (auto* g : s->groups) HIP_TRY(hipStreamSynchronize(g->stream));
+    if (s->use_graph) {
+        if (!s->graphs_ok) {
+            int r = ens_build_graphs<T>(s);
+            if (r) return r;
+        }
+        HIP_TRY(hipGraphLaunch(s->graph[0], s->main));
+        for (int i = 0; i < nt - 1; i++) HIP_TRY(hipGraphLaunch(s->graph[1], s->main));
+        HIP_TRY(hipGraphLaunch(s->graph[2], s->main));
+    } else {
+        ens_issue<T>(s, 0);
+        for (int i = 0; i < nt - 1; i++) ens_issue<T>(s, 1);
+        ens_issue<T>(s, 2);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->main));
+    for (auto* g : s->groups) g->it += (unsigned)nt;
+    return 0;
+}
+
+// Times `steps` real-time steps of one engine for each launcher variant
+// (bit 0: z passes in LDS tiles instead of wave registers, bit 1: half-width
+// strided tiles) and keeps the fastest.  The state is saved and restored.
+template <typename T>
+int autotune_t(gpe_engine_s* e, int steps, int* chosen) {
+    if (steps < 1) steps = 1;
+    const size_t bytes = e->n * e->batch * sizeof(cplx<T>);
+    void* save = nullptr;
+    HIP_TRY(hipMalloc(&save, bytes));
+    const unsigned it0 = e->it;
+    hipEvent_t t0, t1;
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
+    int best = 0;
+    float best_ms = 0.f;
+    int err = 0;
+    (void)hipMemcpyAsync(save, e->A, bytes, hipMemcpyDeviceToDevice, e->stream);
+    const double beta = e->beta;
+    for (int v = 0; v < 4 && !err; v++) {
+        e->variant = v;
+        StepPlan sp;
+        // one untimed step, then the timed ones (real-time schedule in both time modes)
+        e->beta = 0.0;
+        issue_open<T>(e, sp);
+        issue_step<T>(e, sp, true);
+        (void)hipEventRecord(t0, e->stream);
+        issue_open<T>(e, sp);
+        for (int i = 0; i < steps; i++) issue_step<T>(e, sp, i == steps - 1);
+        (void)hipEventRecord(t1, e->stream);
+        e->beta = beta;
+        hipError_t h = hipEventSynchronize(t1);
+        if (h != hipSuccess) { err = (int)h; break; }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        if (v == 0 || ms < best_ms) { best = v; best_ms = ms; }
+    }
+    e->beta = beta;
+    (void)hipMemcpyAsync(e->A, save, bytes, hipMemcpyDeviceToDevice, e->stream);
+    (void)hipStreamSynchronize(e->stream);
+    (void)hipFree(save);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    e->it = it0;
+    e->variant = best;
+    if (chosen) *chosen = best;
+    return err;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpe_engine_autotune(gpe_engine_t e, int steps, int* variant) {
+    if (!e) return GPE_ERR_NO_ENGINE;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return DISPATCH(e, autotune_t, e, steps, variant);
+}
+
+int gpe_engine_set_variant(gpe_engine_t e, int variant) {
+    if (!e) return GPE_ERR_NO_ENGINE;
+    if (variant < 0 || variant > 3) return GPE_ERR_ARG;
+    e->variant = variant;
+    return 0;
+}
+
+int gpe_ensemble_create(gpe_ensemble_t* out, int ninst, const int* dims, int interactions, int vext, int gpe_for,
+                        int precision, double alpha, double beta, double dt, double npart) {
+    if (!out || ninst < 1 || !dims) return GPE_ERR_ARG;
+    *out = nullptr;
+    // group lattices by shape, in order of first appearance
+    std::vector<std::array<int, 3>> shapes;
+    std::vector<size_t> counts;
+    std::vector<int> grp(ninst);
+    std::vector<size_t> slot(ninst);
+    for (int i = 0; i < ninst; i++) {
+        std::array<int, 3> d = {dims[3 * i], dims[3 * i + 1], dims[3 * i + 2]};
+        size_t g = 0;
+        while (g < shapes.size() && shapes[g] != d) g++;
+        if (g == shapes.size()) {
+            shapes.push_back(d);
+            counts.push_back(0);
+        }
+        grp[i] = (int)g;
+        slot[i] = counts[g]++;
+    }
+    gpe_ensemble_s* s = new gpe_ensemble_s();
+    s->precision = precision;
+    s->inst_group = grp;
+    s->inst_slot = slot;
+    for (size_t g = 0; g < shapes.size(); g++) {
+        gpe_config_t cfg{shapes[g][0], shapes[g][1], shapes[g][2], interactions, vext, gpe_for, precision};
+        gpe_engine_s* e = nullptr;
+        const int r = create_impl(&e, &cfg, alpha, beta, dt, npart, counts[g]);
+        if (r) {
+            ens_free(s);
+            return r;
+        }
+        s->groups.push_back(e);
+    }
+    hipError_t err = hipStreamCreateWithFlags(&s->main, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&s->fork, hipEventDisableTiming);
+    for (size_t g = 0; g < shapes.size() && err == hipSuccess; g++) {
+        hipEvent_t ev;
+        err = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (err == hipSuccess) s->join.push_back(ev);
+    }
+    if (err != hipSuccess) {
+        ens_free(s);
+        return (int)err;
+    }
+    *out = s;
+    return 0;
+}
+
+int gpe_ensemble_destroy(gpe_ensemble_t s) {
+    if (!s) return GPE_ERR_NO_ENGINE;
+    if (s->main) (void)hipStreamSynchronize(s->main);
+    ens_free(s);
+    return 0;
+}
+
+int gpe_ensemble_set_user_params(gpe_ensemble_t s, int size, const double* params) {
+    if (!s) return GPE_ERR_NO_ENGINE;
+    ens_drop_graphs(s);
+    for (auto* g : s->groups) {
+        const int r = gpe_engine_set_user_params(g, size, params);
+        if (r) return r;
+    }
+    return 0;
+}
+
+int gpe_ensemble_set_quantum_friction_coeff(gpe_ensemble_t s, double qf) {
+    if (!s) return GPE_ERR_NO_ENGINE;
+    ens_drop_graphs(s);
+    for (auto* g : s->groups) {
+        const int r = gpe_engine_set_quantum_friction_coeff(g, qf);
+        if (r) return r;
+    }
+    return 0;
+}
+
+int gpe_ensemble_set_psi(gpe_ensemble_t s, int inst, const cuCplx* psi) {
+    if (!s) return GPE_ERR_NO_ENGINE;
+    if (inst < 0 || inst >= (int)s->inst_group.size() || !psi) return GPE_ERR_ARG;
+    gpe_engine_s* g = s->groups[s->inst_group[inst]];
+    g->t0 = 0.0;
+    g->it = 0;
+    return DISPATCH(g, set_psi_t, g, psi, s->inst_slot[inst]);
+}
+
+int gpe_ensemble_get_psi(gpe_ensemble_t s, int inst, double* t, cuCplx* psi) {
+    if (!s) return GPE_ERR_NO_ENGINE;
+    if (inst < 0 || inst >= (int)s->inst_group.size() || !psi) return GPE_ERR_ARG;
+    gpe_engine_s* g = s->groups[s->inst_group[inst]];
+    if (t) *t = g->t0 + g->dt * g->it;
+    return DISPATCH(g, get_psi_t, g, psi, s->inst_slot[inst]);
+}
+
+int gpe_ensemble_evolve(gpe_ensemble_t s, int nt) {
+    if (!s) return GPE_ERR_NO_ENGINE;
+    if (nt < 0) return GPE_ERR_ARG;
+    return s->precision == GPE_PRECISION_C64 ? ens_evolve_t<float>(s, nt) : ens_evolve_t<double>(s, nt);
+}
+
+int gpe_ensemble_set_graph(gpe_ensemble_t s, int on) {
+    if (!s) return GPE_ERR_NO_ENGINE;
+    s->use_graph = (on != 0);
+    return 0;
+}
+
+int gpe_ensemble_autotune(gpe_ensemble_t s, int steps) {
+    if (!s) return GPE_ERR_NO_ENGINE;
+    ens_drop_graphs(s);
+    for (auto* g : s->groups) {
+        const int r = gpe_engine_autotune(g, steps, nullptr);
+        if (r) return r;
+    }
+    return 0;
+}
+
+int gpe_ensemble_groups(gpe_ensemble_t s, int* ngroups) {
+    if (!s) return GPE_ERR_NO_ENGINE;
+    if (!ngroups) return GPE_ERR_ARG;
+    *ngroups = (int)s->groups.size();
+    return 0;
+}
+
+int gpe_ensemble_group_info(gpe_ensemble_t s, int group, int* nx, int* ny, int* nz, int* count, int* variant) {
+    if (!s) return GPE_ERR_NO_ENGINE;
+    if (group < 0 || group >= (int)s->groups.size()) return GPE_ERR_ARG;
+    const gpe_engine_s* g = s->groups[group];
+    if (nx) *nx = g->nx;
+    if (ny) *ny = g->ny;
+    if (nz) *nz = g->nz;
+    if (count) *count = (int)g->batch;
+    if (variant) *variant = g->variant;
+    return 0;
+}
+
+int gpe_ensemble_instance(gpe_ensemble_t s, int inst, int* group, int* slot) {
+    if (!s) return GPE_ERR_NO_ENGINE;
+    if (inst < 0 || inst >= (int)s->inst_group.size()) return GPE_ERR_ARG;
+    if (group) *group = s->inst_group[inst];
+    if (slot) *slot = (int)s->inst_slot[inst];
+    return 0;
+}
+
+int gpe_ensemble_mean_density(gpe_ensemble_t s, int group, double* out) {
+    if (!s) return GPE_ERR_NO_ENGINE;
+    if (group < 0 || group >= (int)s->groups.size() || !out) return GPE_ERR_ARG;
+    gpe_engine_s* g = s->groups[group];
+    int r = ensure_rout(g);
+    if (r) return r;
+    if (g->precision == GPE_PRECISION_C64)
+        launch_mean_density<float>((const cplx<float>*)g->A, (unsigned)g->n, (unsigned)g->batch, g->gamma, g->rout,
+                                   g->stream);
+    else
+        launch_mean_density<double>((const cplx<double>*)g->A, (unsigned)g->n, (unsigned)g->batch, g->gamma, g->rout,
+                                    g->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, g->rout, g->n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return 0;
 }
 
 }  // extern "C"

@@ -300,6 +300,7 @@ enum MidMode {
     MID_KIN_SUM = 7,     // sum |k|^2 |psi_k|^2 (kinetic energy by Parseval)
     MID_KA = 8,          // k_a / (gamma N) (orginal 1125-1162)
     MID_OVERLAP_RE = 9,  // Re(conj psi * v) -> real output (orginal 1067-1074)
+    MID_EXPTC = 10,      // conj(Kx Ky Kz)/N: the adjoint of the kinetic factor
 };
 enum StoreMode { STORE_PLAIN = 0, STORE_NONE = 1 };
 enum PassFlags {
@@ -323,6 +324,7 @@ struct PassArgs {
     int dir1, dir2;        // -1 / +1; dir2 == 0: single transform
     int load_mode, mid_mode, store_mode, flags;
     int nred;              // number of FP64 sums produced (0..3)
+    int variant;           // launcher choice: bit 0 z passes in LDS tiles, bit 1 half-width strided tiles
     const cplx<T>* src;
     cplx<T>* dst;
     const cplx<T>* tw;     // exp(-2 pi i m / N), m < N
@@ -416,20 +418,27 @@ __device__ __forceinline__ double dens64(double g, double x, double y) { return 
 template <typename T>
 __device__ __forceinline__ cplx<T> mid_op_lean(const PassArgs<T>& a, const Coord& c, cplx<T> v) {
     switch (a.mid_mode) {
-    case MID_EXPT: {
+    case MID_EXPT:
+    case MID_EXPTC: {
+        const bool cj = (a.mid_mode == MID_EXPTC);
         if constexpr (sizeof(T) == 4) {
             if (a.axis == 0 && a.kx_hl) {
                 // complex64 on the x axis: two compensated products with K_x and
                 // the per-pencil K_y K_z (hi + lo float pairs of FP64 values)
                 const unsigned yz = c.iy * (unsigned)a.ph.nz + c.iz;
-                v = cmul_comp(v, a.kx_hl[2 * c.ix], a.kx_hl[2 * c.ix + 1]);
-                return cmul_comp(v, a.kyz_hl[2 * yz], a.kyz_hl[2 * yz + 1]);
+                cplx<float> h1 = a.kx_hl[2 * c.ix], l1 = a.kx_hl[2 * c.ix + 1];
+                cplx<float> h2 = a.kyz_hl[2 * yz], l2 = a.kyz_hl[2 * yz + 1];
+                if (cj) { h1.y = -h1.y; l1.y = -l1.y; h2.y = -h2.y; l2.y = -l2.y; }
+                v = cmul_comp(v, h1, l1);
+                return cmul_comp(v, h2, l2);
             }
         }
         cplx<double> w = mk((double)v.x, (double)v.y);
-        w = cmul(w, a.Kx[c.ix]);
-        w = cmul(w, a.Ky[c.iy]);
-        w = cmul(w, a.Kz[c.iz]);
+        cplx<double> kx = a.Kx[c.ix], ky = a.Ky[c.iy], kz = a.Kz[c.iz];
+        if (cj) { kx.y = -kx.y; ky.y = -ky.y; kz.y = -kz.y; }
+        w = cmul(w, kx);
+        w = cmul(w, ky);
+        w = cmul(w, kz);
         return mk((T)w.x, (T)w.y);
     }
     case MID_K2: {
@@ -559,7 +568,7 @@ __device__ __forceinline__ cplx<T> mid_op_m(const PassArgs<T>& a, const ElemIn<T
         const cplx<T> p = e.aux;
         a.rout[e.c.off] = (double)(p.x * v.x + p.y * v.y);
         return v;
-    } else if constexpr (MM == MID_EXPT || MM == MID_K2 || MM == MID_KA) {
+    } else if constexpr (MM == MID_EXPT || MM == MID_EXPTC || MM == MID_K2 || MM == MID_KA) {
         return mid_op_lean(a, e.c, v);
     } else {
         return v;
@@ -576,6 +585,7 @@ __device__ __forceinline__ cplx<T> mid_op2(const PassArgs<T>& a, const ElemIn<T>
     case MID_KIN_SUM: return mid_op_m<MID_KIN_SUM>(a, e, v, acc);
     case MID_OVERLAP_RE: return mid_op_m<MID_OVERLAP_RE>(a, e, v, acc);
     case MID_EXPT:
+    case MID_EXPTC:
     case MID_K2:
     case MID_KA: return mid_op_lean(a, e.c, v);
     default: return v;
@@ -836,10 +846,16 @@ enum PassShape { SHAPE_F = 0, SHAPE_I = 1, SHAPE_FI = 2, SHAPE_IF = 3 };
 
 // Lean passes (plain load and store, at most a table factor in the middle):
 // everything between the first load and the last store stays in registers or LDS.
-template <typename T, int N, bool CONTIG, int S1, int S2>
+// tile width of a lean pass; WD = 2 halves it (an autotuner choice for strided passes)
+template <typename T, int N, int WD>
+constexpr int lean_w() {
+    return tile_w<T, N>() / WD > 0 ? tile_w<T, N>() / WD : 1;
+}
+
+template <typename T, int N, bool CONTIG, int S1, int S2, int WD>
 __device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, const cplx<T>* tw,
                                           const cplx<T>* twlo) {
-    constexpr int W = tile_w<T, N>();
+    constexpr int W = lean_w<T, N, WD>();
     constexpr FftPlan P = plan_of(N, false);
     constexpr int NSG = P.ns;
     constexpr int RL = P.r[NSG - 1];
@@ -1064,7 +1080,7 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
     }
 }
 
-template <typename T, int N, bool CONTIG, bool FULL, int SHAPE>
+template <typename T, int N, bool CONTIG, bool FULL, int SHAPE, int WD = 1>
 __global__ __launch_bounds__(kThreads) void fft_pass_kernel(const PassArgs<T> a) {
     __shared__ cplx<T> tw[N];
     __shared__ cplx<T> twlo[sizeof(T) == 4 ? N : 1];
@@ -1077,9 +1093,9 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(const PassArgs<T> a)
     constexpr int S1 = (SHAPE == SHAPE_F || SHAPE == SHAPE_FI) ? -1 : 1;
     constexpr int S2 = (SHAPE == SHAPE_FI) ? 1 : (SHAPE == SHAPE_IF) ? -1 : 0;
     if constexpr (!FULL) {
-        constexpr int W = tile_w<T, N>();
+        constexpr int W = lean_w<T, N, WD>();
         __shared__ cplx<T> lds[lds_elems<N, W, CONTIG>()];
-        lean_body<T, N, CONTIG, S1, S2>(a, lds, tw, twlo);
+        lean_body<T, N, CONTIG, S1, S2, WD>(a, lds, tw, twlo);
     } else {
         __shared__ FullLds<T, N, CONTIG> L;
         __shared__ double red[kThreads / 64];
@@ -1105,7 +1121,8 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(const PassArgs<T> a)
 template <typename T>
 inline bool pass_is_lean(const PassArgs<T>& a) {
     return a.load_mode == LOAD_PLAIN && a.flags == 0 && a.nred == 0 &&
-           (a.mid_mode == MID_NONE || a.mid_mode == MID_EXPT || a.mid_mode == MID_K2 || a.mid_mode == MID_KA);
+           (a.mid_mode == MID_NONE || a.mid_mode == MID_EXPT || a.mid_mode == MID_EXPTC || a.mid_mode == MID_K2 ||
+            a.mid_mode == MID_KA);
 }
 
 inline int pass_shape(int dir1, int dir2) {

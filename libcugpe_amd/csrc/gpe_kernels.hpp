@@ -49,6 +49,11 @@ void launch_scale(cplx<T>* A, unsigned n, const double* scal, hipStream_t s);
 template <typename T>
 void launch_density(const cplx<T>* A, unsigned n, double gamma, double* out, hipStream_t s);
 
+// out = (1/batch) sum_b gamma |psi_b|^2 over `batch` lattices of n points
+// stored one after another (FP64, summed in lattice order)
+template <typename T>
+void launch_mean_density(const cplx<T>* A, unsigned n, unsigned batch, double gamma, double* out, hipStream_t s);
+
 // plaquette windings w[3][n]; count of non-zero windings summed into partials, nb returned
 template <typename T>
 int launch_windings(const cplx<T>* A, int nx, int ny, int nz, double rho_min, int8_t* w, double* partials,
@@ -70,5 +75,33 @@ void launch_to_c128(const cplx<T>* in, cplx<double>* out, unsigned n, hipStream_
 void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 constexpr int kMaxReduceBlocks = 1024;
+
+// discrete adjoint of the real-time split step (gpe_adjoint.hip)
+struct AdjArgs {
+    int interactions;
+    double gamma;
+    double up[kMaxUserParams];
+    double cx, cy;          // step coefficient c
+    size_t n;
+    const double* vx;       // V_ext per point (FP64)
+    const void* psi0;       // step input and output states
+    const void* psi1;
+    void* phi;              // work: phi1 / phi2 / phi3
+    void* lam;              // adjoint state
+    void* lam0;             // work: the part of the new adjoint state carried to P4
+    double* vb;             // potential sensitivities (FP64)
+    double* grad;           // dL/dV_ext accumulated (FP64)
+};
+// part 1..4: the pointwise kernels P1..P4 of one backward step
+template <typename T>
+void launch_adjoint_step(const AdjArgs& a, int part, hipStream_t s);
+template <typename T>
+void launch_adjoint_vfill(const Phys<T>& ph, double* out, size_t n, hipStream_t s);
+// partials [4][nb]: Re, Im of sum conj(tgt) psi, sum |a0|^2, sum |tgt|^2; nb returned
+template <typename T>
+int launch_adjoint_overlap(const void* psi, const void* tgt, const void* a0, size_t n, double* partials,
+                           hipStream_t s);
+template <typename T>
+void launch_adjoint_lam_init(void* lam, const void* tgt, double fr, double fi, size_t n, hipStream_t s);
 
 }  // namespace gpe

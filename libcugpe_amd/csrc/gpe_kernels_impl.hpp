@@ -71,6 +71,21 @@ __global__ __launch_bounds__(kThreads) void density_kernel(const cplx<T>* __rest
 }
 
 template <typename T>
+__global__ __launch_bounds__(kThreads) void mean_density_kernel(const cplx<T>* __restrict__ A, unsigned n,
+                                                                unsigned batch, double gamma,
+                                                                double* __restrict__ out) {
+    const unsigned i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) {
+        double acc = 0.0;
+        for (unsigned b = 0; b < batch; b++) {
+            const cplx<T> v = A[(size_t)b * n + i];
+            acc += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+        }
+        out[i] = gamma * acc / (double)batch;
+    }
+}
+
+template <typename T>
 __global__ __launch_bounds__(kThreads) void phase_kernel(const cplx<T>* __restrict__ A, unsigned n,
                                                          double* __restrict__ out) {
     const unsigned i = blockIdx.x * kThreads + threadIdx.x;
@@ -223,6 +238,11 @@ static inline dim3 ew_grid(unsigned n) { return dim3((n + kThreads - 1) / kThrea
     template <>                                                                                                   \
     void launch_density<T>(const cplx<T>* A, unsigned n, double gamma, double* out, hipStream_t s) {              \
         hipLaunchKernelGGL((density_kernel<T>), ew_grid(n), dim3(kThreads), 0, s, A, n, gamma, out);              \
+    }                                                                                                             \
+    template <>                                                                                                   \
+    void launch_mean_density<T>(const cplx<T>* A, unsigned n, unsigned batch, double gamma, double* out,          \
+                                hipStream_t s) {                                                                  \
+        hipLaunchKernelGGL((mean_density_kernel<T>), ew_grid(n), dim3(kThreads), 0, s, A, n, batch, gamma, out);  \
     }                                                                                                             \
     template <>                                                                                                   \
     int launch_windings<T>(const cplx<T>* A, int nx, int ny, int nz, double rho_min, int8_t* w, double* partials, \

@@ -80,6 +80,23 @@ def lib():
             "gpe_engine_profile": [_p, _i, _i],
             "gpe_engine_profile_read": [_p, _dp, ctypes.POINTER(ctypes.c_long)],
             "gpe_engine_copy_benchmark": [_p, _dp],
+            "gpe_engine_autotune": [_p, _i, ctypes.POINTER(_i)],
+            "gpe_engine_adjoint_forward": [_p, _i, _p, _i, _dp],
+            "gpe_engine_adjoint_backward": [_p, _p],
+            "gpe_engine_set_variant": [_p, _i],
+            "gpe_ensemble_create": [ctypes.POINTER(_p), _i, _p, _i, _i, _i, _i, _d, _d, _d, _d],
+            "gpe_ensemble_destroy": [_p],
+            "gpe_ensemble_set_user_params": [_p, _i, _p],
+            "gpe_ensemble_set_quantum_friction_coeff": [_p, _d],
+            "gpe_ensemble_set_psi": [_p, _i, _p],
+            "gpe_ensemble_get_psi": [_p, _i, _dp, _p],
+            "gpe_ensemble_evolve": [_p, _i],
+            "gpe_ensemble_set_graph": [_p, _i],
+            "gpe_ensemble_autotune": [_p, _i],
+            "gpe_ensemble_groups": [_p, ctypes.POINTER(_i)],
+            "gpe_ensemble_group_info": [_p, _i] + [ctypes.POINTER(_i)] * 5,
+            "gpe_ensemble_instance": [_p, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)],
+            "gpe_ensemble_mean_density": [_p, _i, _p],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -277,8 +294,109 @@ class Engine:
         _check(lib().gpe_engine_copy_benchmark(self._h, ctypes.byref(ms)), "gpe_engine_copy_benchmark")
         return ms.value
 
+    def adjoint_forward(self, nt, psi_target, ckpt=0):
+        """nt real-time steps keeping checkpoints; returns
+        L = 1 - |<psi_t|psi(T)>|^2 / (|psi_t|^2 |psi(0)|^2)."""
+        pt = np.ascontiguousarray(np.asarray(psi_target, dtype=np.complex128))
+        if pt.shape != self.shape:
+            raise ValueError("target shape %s, lattice %s" % (pt.shape, self.shape))
+        loss = ctypes.c_double()
+        _check(lib().gpe_engine_adjoint_forward(self._h, int(nt), _ptr(pt), int(ckpt), ctypes.byref(loss)),
+               "gpe_engine_adjoint_forward")
+        return loss.value
+
+    def adjoint_backward(self):
+        """dL/dV_ext at every lattice point for the last adjoint_forward."""
+        g = np.empty(self.shape, dtype=np.float64)
+        _check(lib().gpe_engine_adjoint_backward(self._h, _ptr(g)), "gpe_engine_adjoint_backward")
+        return g
+
+    def autotune(self, steps=3):
+        """Time each launcher variant over `steps` real-time steps, keep the fastest; returns it."""
+        v = ctypes.c_int()
+        _check(lib().gpe_engine_autotune(self._h, int(steps), ctypes.byref(v)), "gpe_engine_autotune")
+        return v.value
+
+    def set_variant(self, variant):
+        _check(lib().gpe_engine_set_variant(self._h, int(variant)), "gpe_engine_set_variant")
+
     def profile_read(self):
         ms = ctypes.c_double()
         cnt = ctypes.c_long()
         _check(lib().gpe_engine_profile_read(self._h, ctypes.byref(ms), ctypes.byref(cnt)), "gpe_engine_profile_read")
         return ms.value, cnt.value
+
+
+class Ensemble:
+    """Many independent lattices advanced together in real time
+    (gpe_ensemble_* of include/gpe_engine.h).  `shapes` lists one (nx, ny, nz)
+    per lattice; lattices of one shape form a group."""
+
+    def __init__(self, shapes, alpha, beta, dt, npart, interactions=UFG, vext=HARMONIC, gpe_for=0, precision=C64):
+        self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        dims = np.ascontiguousarray(np.array(self.shapes, dtype=np.int32).reshape(-1))
+        h = ctypes.c_void_p()
+        _check(lib().gpe_ensemble_create(ctypes.byref(h), len(self.shapes), _ptr(dims), interactions, vext, gpe_for,
+                                         precision, alpha, beta, dt, npart), "gpe_ensemble_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _check(lib().gpe_ensemble_destroy(self.h), "gpe_ensemble_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_user_params(self, params):
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64))
+        _check(lib().gpe_ensemble_set_user_params(self.h, len(p), _ptr(p)), "gpe_ensemble_set_user_params")
+
+    def set_quantum_friction_coeff(self, qf):
+        _check(lib().gpe_ensemble_set_quantum_friction_coeff(self.h, qf), "gpe_ensemble_set_quantum_friction_coeff")
+
+    def set_psi(self, inst, psi):
+        a = np.ascontiguousarray(np.asarray(psi, dtype=np.complex128))
+        if a.shape != self.shapes[inst]:
+            raise ValueError("psi shape %s, lattice %d is %s" % (a.shape, inst, self.shapes[inst]))
+        _check(lib().gpe_ensemble_set_psi(self.h, inst, _ptr(a)), "gpe_ensemble_set_psi")
+
+    def get_psi(self, inst):
+        a = np.empty(self.shapes[inst], dtype=np.complex128)
+        t = ctypes.c_double()
+        _check(lib().gpe_ensemble_get_psi(self.h, inst, ctypes.byref(t), _ptr(a)), "gpe_ensemble_get_psi")
+        return a, t.value
+
+    def evolve(self, nt):
+        _check(lib().gpe_ensemble_evolve(self.h, int(nt)), "gpe_ensemble_evolve")
+
+    def set_graph(self, on):
+        _check(lib().gpe_ensemble_set_graph(self.h, 1 if on else 0), "gpe_ensemble_set_graph")
+
+    def autotune(self, steps=3):
+        _check(lib().gpe_ensemble_autotune(self.h, int(steps)), "gpe_ensemble_autotune")
+
+    def groups(self):
+        """[(shape, count, variant)] per group."""
+        n = ctypes.c_int()
+        _check(lib().gpe_ensemble_groups(self.h, ctypes.byref(n)), "gpe_ensemble_groups")
+        out = []
+        for g in range(n.value):
+            v = [ctypes.c_int() for _ in range(5)]
+            _check(lib().gpe_ensemble_group_info(self.h, g, *[ctypes.byref(x) for x in v]), "gpe_ensemble_group_info")
+            out.append(((v[0].value, v[1].value, v[2].value), v[3].value, v[4].value))
+        return out
+
+    def instance(self, inst):
+        g, s = ctypes.c_int(), ctypes.c_int()
+        _check(lib().gpe_ensemble_instance(self.h, inst, ctypes.byref(g), ctypes.byref(s)), "gpe_ensemble_instance")
+        return g.value, s.value
+
+    def mean_density(self, group):
+        shape = self.groups()[group][0]
+        out = np.empty(shape, dtype=np.float64)
+        _check(lib().gpe_ensemble_mean_density(self.h, group, _ptr(out)), "gpe_ensemble_mean_density")
+        return out

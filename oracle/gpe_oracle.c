@@ -799,3 +799,210 @@ long gpeo_plaquette_windings(int nx, int ny, int nz, const double *psi64, const 
     }
     return count;
 }
+
+/* ------------------------------------------------------------------ adjoint
+ *
+ * Gradient of the final-time overlap objective
+ *     L = 1 - |<psi_t|psi(T)>|^2 / (|psi_t|^2 |psi(0)|^2)
+ * with respect to V_ext at every lattice point, for T real-time split steps
+ * of one_step above (no quantum friction, no phase pinning).  No counterpart
+ * in the reference.  The discrete adjoint of one step, with the convention
+ * dL = 2 Re sum conj(lambda) dpsi and c the step coefficient, is
+ *
+ *   forward:  V0 = Vext + g(rho(psi0)), E0 = exp(c V0), phi1 = psi0 E0,
+ *             phi3 = K(phi1) E0, V1 = (V0 + Vext + g(rho(phi3)))/2,
+ *             W = exp(c V1), chi = psi0 W, psi1 = K(chi) W
+ *   backward: lambda_xi = conj(W) lambda1,  Vb1 = 2 Re(conj(lambda1) psi1 c)
+ *             lambda_chi = K^+(lambda_xi),  Vb1 += 2 Re(conj(lambda_chi) chi c)
+ *             lambda0 = conj(W) lambda_chi
+ *             dL/dVext += Vb1/2,  Vb0 = Vb1/2,
+ *             lambda_phi3 = (Vb1/2) g'(rho3) gamma phi3,  Vb0 += 2 Re(conj(lambda_phi3) phi3 c)
+ *             lambda_phi1 = K^+(conj(E0) lambda_phi3), Vb0 += 2 Re(conj(lambda_phi1) phi1 c)
+ *             lambda0 += conj(E0) lambda_phi1 + gamma g'(rho0) Vb0 psi0,  dL/dVext += Vb0
+ * where K = F^-1 diag(Kx Ky Kz) F is the kinetic step and K^+ the same with
+ * conj(Kx Ky Kz) (the unnormalised F and F^-1 are each other's adjoints), and
+ * g = dEDF/drho with g' its derivative.  States are kept every `ckpt` steps
+ * and each segment is recomputed once on the way back.
+ */
+
+static double dgdrho(const gpeo_t *e, double rho)
+{
+    switch (e->interactions) {
+    case 0: /* g = 0.37 (3 pi^2 rho)^(2/3) / 2  ->  g' = (2/3) g / rho */
+        return rho > 0.0 ? (2.0 / 3.0) * dedfdn(e, rho) / rho : 0.0;
+    case -1:
+        return 4 * M_PI * e->up[3];
+    case -2:
+        return 0.0;
+    default:
+        return NAN;
+    }
+}
+
+static void kinetic_step(gpeo_t *e, cd *a, int adjoint)
+{
+    fft3(e, a, -1);
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++)
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                cd w = a[i];
+                cd kx = e->Kx[ix], ky = e->Ky[iy], kz = e->Kz[iz];
+                if (adjoint) { kx.im = -kx.im; ky.im = -ky.im; kz.im = -kz.im; }
+                w = cmul(w, kx);
+                w = cmul(w, ky);
+                w = cmul(w, kz);
+                a[i] = w;
+            }
+    fft3(e, a, +1);
+}
+
+static inline cd cconj(cd a) { cd r = {a.re, -a.im}; return r; }
+/* 2 Re(conj(l) y c) */
+static inline double sens(cd l, cd y, cd c) { return 2.0 * cmul(cmul(cconj(l), y), c).re; }
+
+static void backward_step(gpeo_t *e, const cd *psi0, const cd *psi1, cd *lam, double *grad, cd *phi, cd *lam0,
+                          double *vb)
+{
+    const cd c = e->c;
+    const double G = e->gamma;
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++)
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                double V0 = vext_val(e, ix, iy, iz, i) + dedfdn(e, density(e, psi0[i]));
+                phi[i] = cmul(psi0[i], cexp_(cscale(c, V0)));
+            }
+    kinetic_step(e, phi, 0); /* phi2 */
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++)
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                double vx = vext_val(e, ix, iy, iz, i);
+                double V0 = vx + dedfdn(e, density(e, psi0[i]));
+                cd phi3 = cmul(phi[i], cexp_(cscale(c, V0)));
+                double V1 = 0.5 * (V0 + vx + dedfdn(e, density(e, phi3)));
+                cd W = cexp_(cscale(c, V1));
+                vb[i] = sens(lam[i], psi1[i], c);
+                lam[i] = cmul(cconj(W), lam[i]);
+                phi[i] = phi3;
+            }
+    kinetic_step(e, lam, 1); /* lambda_chi */
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++)
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                double vx = vext_val(e, ix, iy, iz, i);
+                double V0 = vx + dedfdn(e, density(e, psi0[i]));
+                cd E0 = cexp_(cscale(c, V0));
+                cd phi3 = phi[i];
+                double r3 = density(e, phi3);
+                double V1 = 0.5 * (V0 + vx + dedfdn(e, r3));
+                cd W = cexp_(cscale(c, V1));
+                cd chi = cmul(psi0[i], W);
+                lam0[i] = cmul(cconj(W), lam[i]);
+                double V1b = vb[i] + sens(lam[i], chi, c);
+                grad[i] += 0.5 * V1b;
+                double V0b = 0.5 * V1b;
+                cd lp3 = cscale(phi3, 0.5 * V1b * dgdrho(e, r3) * G);
+                V0b += sens(lp3, phi3, c);
+                lam[i] = cmul(cconj(E0), lp3);
+                vb[i] = V0b;
+            }
+    kinetic_step(e, lam, 1); /* lambda_phi1 */
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++)
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                double r0 = density(e, psi0[i]);
+                double V0 = vext_val(e, ix, iy, iz, i) + dedfdn(e, r0);
+                cd E0 = cexp_(cscale(c, V0));
+                cd phi1 = cmul(psi0[i], E0);
+                cd l0 = lam0[i];
+                cd t = cmul(cconj(E0), lam[i]);
+                l0.re += t.re;
+                l0.im += t.im;
+                double V0b = vb[i] + sens(lam[i], phi1, c);
+                grad[i] += V0b;
+                double f = G * dgdrho(e, r0) * V0b;
+                l0.re += f * psi0[i].re;
+                l0.im += f * psi0[i].im;
+                lam[i] = l0;
+            }
+}
+
+int gpeo_adjoint_overlap(gpeo_t *e, int nt, const double *psi_target, int ckpt, double *loss, double *grad)
+{
+    if (nt < 1 || !psi_target || !grad) return -12;
+    if (e->beta != 0.0 || e->qf != 0.0 || isnan(dgdrho(e, 1.0))) return -12;
+    const size_t n = e->n;
+    const int C = ckpt > 0 ? ckpt : (int)ceil(sqrt((double)nt));
+    const int nck = (nt + C - 1) / C;
+    const cd *pt = (const cd *)psi_target;
+    cd *ck = (cd *)malloc(sizeof(cd) * n * (size_t)nck);
+    cd *seg = (cd *)malloc(sizeof(cd) * n * (size_t)(C + 1));
+    cd *lam = (cd *)malloc(sizeof(cd) * n);
+    cd *phi = (cd *)malloc(sizeof(cd) * n);
+    cd *lam0 = (cd *)malloc(sizeof(cd) * n);
+    double *vb = (double *)malloc(sizeof(double) * n);
+    double *pl = plane_buf(e);
+    if (!ck || !seg || !lam || !phi || !lam0 || !vb || !pl) {
+        free(ck); free(seg); free(lam); free(phi); free(lam0); free(vb); free(pl);
+        return -12;
+    }
+    const unsigned it0 = e->it;
+    for (int k = 0; k < nt; k++) {
+        if (k % C == 0) memcpy(ck + (size_t)(k / C) * n, e->psi, sizeof(cd) * n);
+        one_step(e, PIN_NONE, NULL);
+    }
+    /* overlap and norms, summed per x plane in a fixed order */
+    double ore = 0, oim = 0, n0 = 0, ntg = 0;
+    for (int ix = 0; ix < e->nx; ix++) {
+        double a = 0, b = 0, c0 = 0, d = 0;
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                cd p = e->psi[i], t = pt[i], q = ck[i];
+                a += t.re * p.re + t.im * p.im;
+                b += t.re * p.im - t.im * p.re;
+                c0 += q.re * q.re + q.im * q.im;
+                d += t.re * t.re + t.im * t.im;
+            }
+        ore += a;
+        oim += b;
+        n0 += c0;
+        ntg += d;
+    }
+    const double s = 1.0 / (n0 * ntg);
+    *loss = 1.0 - (ore * ore + oim * oim) * s;
+    /* lambda_T = -s O psi_t */
+    for (size_t i = 0; i < n; i++) {
+        cd t = pt[i];
+        lam[i].re = -s * (ore * t.re - oim * t.im);
+        lam[i].im = -s * (ore * t.im + oim * t.re);
+        grad[i] = 0.0;
+    }
+    cd *final_psi = (cd *)malloc(sizeof(cd) * n);
+    memcpy(final_psi, e->psi, sizeof(cd) * n);
+    for (int sgi = nck - 1; sgi >= 0; sgi--) {
+        const int k0 = sgi * C;
+        const int k1 = (k0 + C < nt) ? k0 + C : nt;
+        memcpy(e->psi, ck + (size_t)sgi * n, sizeof(cd) * n);
+        memcpy(seg, e->psi, sizeof(cd) * n);
+        for (int j = 1; j <= k1 - k0; j++) {
+            one_step(e, PIN_NONE, NULL);
+            memcpy(seg + (size_t)j * n, e->psi, sizeof(cd) * n);
+        }
+        for (int k = k1 - 1; k >= k0; k--)
+            backward_step(e, seg + (size_t)(k - k0) * n, seg + (size_t)(k - k0 + 1) * n, lam, grad, phi, lam0, vb);
+    }
+    memcpy(e->psi, final_psi, sizeof(cd) * n);
+    e->it = it0 + (unsigned)nt;
+    free(final_psi); free(ck); free(seg); free(lam); free(phi); free(lam0); free(vb); free(pl);
+    return 0;
+}

@@ -88,6 +88,14 @@ void gpeo_unit_root(long m, long n, double *re, double *im);
 long gpeo_plaquette_windings(int nx, int ny, int nz, const double *psi64,
                              const float *psi32, double rho_min, int8_t *w);
 
+/* Gradient dL/dV_ext[ixyz] (grad, N doubles) of
+ *   L = 1 - |<psi_t|psi(T)>|^2 / (|psi_t|^2 |psi(0)|^2)
+ * over nt real-time steps from the current psi (the discrete adjoint of the
+ * split step, states checkpointed every ckpt steps, 0: ceil(sqrt(nt))).
+ * Real time without quantum friction and INTERACTIONS 0, -1 or -2 only
+ * (otherwise -12).  Leaves psi at psi(T). */
+int gpeo_adjoint_overlap(gpeo_t *e, int nt, const double *psi_target, int ckpt, double *loss, double *grad);
+
 int gpeo_num_threads(void);
 
 #ifdef __cplusplus

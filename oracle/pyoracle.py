@@ -66,6 +66,7 @@ def lib():
             ("gpeo_dedfdn_value", [_p, _d, _dp, _dp]),
             ("gpeo_fft3d", [_i, _i, _i, _p, _i]),
             ("gpeo_fft1d", [_i, _p, _i]),
+            ("gpeo_adjoint_overlap", [_p, _i, _p, _i, _dp, _p]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -230,6 +231,16 @@ class OracleEngine:
         t = ctypes.c_double()
         assert lib().gpeo_get_density(self._h, ctypes.byref(t), _ptr(a)) == 0
         return a, t.value
+
+    def adjoint_overlap(self, nt, psi_target, ckpt=0):
+        """(L, dL/dV_ext) of L = 1 - |<psi_t|psi(T)>|^2/(|psi_t|^2 |psi(0)|^2) over nt steps."""
+        pt = np.ascontiguousarray(np.asarray(psi_target, dtype=np.complex128))
+        grad = np.empty(self.shape, dtype=np.float64)
+        loss = ctypes.c_double()
+        rc = lib().gpeo_adjoint_overlap(self._h, int(nt), _ptr(pt), int(ckpt), ctypes.byref(loss), _ptr(grad))
+        if rc != 0:
+            raise ValueError("gpeo_adjoint_overlap: %d" % rc)
+        return loss.value, grad
 
     def get_currents(self):
         js = [np.empty(self.shape, dtype=np.float64) for _ in range(3)]

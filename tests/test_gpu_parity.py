@@ -270,3 +270,25 @@ def test_pinned_variants_match_oracle(oracle, variant, mode):
     # gpe_get_phase: arg psi, kept as the enforced phase
     assert np.max(np.abs(g.get_phase() - o.get_phase())) < 1e-10
     g.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 8, 64), (16, 96, 8)])
+def test_launcher_variants_match_oracle(oracle, variant, shape):
+    """Every launcher choice of the autotuner (z passes in wave registers or LDS
+    tiles, full or half-width strided tiles) follows the oracle's step."""
+    say("variant", variant, shape)
+    for mode in ("rte", "ite"):
+        alpha, beta = (1.0, 0.0) if mode == "rte" else (0.0, 1.0)
+        g, o = pair(oracle, shape, alpha, beta, 0.02, 600.0, E.CONTACT, E.COSINE, [0.1, 2.0, 0.0, 0.02])
+        g.set_variant(variant)
+        psi = seeded_state(shape, 23)
+        for x in (g, o):
+            x.set_psi(psi)
+            x.normalize_psi()
+        mg = g.evolve_qf(4, want_mu=True)
+        mo = o.evolve_qf(4, want_mu=True)
+        assert rel_err(g.get_psi()[0], o.get_psi()[0]) < TOL128, mode
+        if mode == "ite":
+            assert abs(mg - mo) < TOL128 * abs(mo)
+        g.close()
