@@ -204,6 +204,96 @@ def ensemble_bench(E, steps, seq_steps, ref_members, ref_steps):
     }
 
 
+def adjoint_bench(E, T, oracle_steps, fd_bumps=(0, 101, 307)):
+    """64^3 complex64 optimal-control gradient: L = 1 - |<psi_t|psi(T)>|^2 over
+    T real-time steps (dt = 1e-3, unitary Fermi gas), dL/dA for 512 seeded
+    Gaussian bumps of V_ext (amplitude ~ N(0, 0.05), width 1.5) through the
+    torch.autograd wrapper of the engine's adjoint."""
+    import torch
+    from time import perf_counter
+
+    from libcugpe_amd import adjoint as A
+
+    shape = (64, 64, 64)
+    n = int(np.prod(shape))
+    dt_a = 1e-3
+    rng = np.random.default_rng(5)
+    centres = A.bump_centres(shape, 512, 5)
+    amp0 = 0.05 * rng.standard_normal(512)
+    psi0 = (1 + 0.05 * rng.random(shape)) * np.exp(1j * 0.1 * rng.standard_normal(shape))
+    # target: psi0 evolved under another seeded bump potential
+    rt = np.random.default_rng(55)
+    vt = A.gaussian_bumps(torch.tensor(0.05 * rt.standard_normal(512)), A.bump_centres(shape, 512, 55), 1.5,
+                          shape).numpy()
+    g = E.Engine(shape, 1.0, 0.0, dt_a, float(n), E.UFG, E.VARRAY, 0, E.C128)
+    g.set_external_potential(vt)
+    g.set_psi(psi0)
+    g.evolve_qf(T)
+    psit, _ = g.get_psi()
+
+    def loss128(a):
+        V = A.gaussian_bumps(torch.tensor(a), centres, 1.5, shape).numpy()
+        g.set_external_potential(V)
+        g.set_psi(psi0)
+        g.evolve_qf(T)
+        p, _ = g.get_psi()
+        ov = np.vdot(psit, p)
+        return 1.0 - abs(ov) ** 2 / (np.vdot(psit, psit).real * np.vdot(psi0, psi0).real)
+
+    eng = E.Engine(shape, 1.0, 0.0, dt_a, float(n), E.UFG, E.VARRAY, 0, E.C64)
+    amp = torch.tensor(amp0, requires_grad=True)
+    A.overlap_loss(A.gaussian_bumps(amp, centres, 1.5, shape), eng, psi0, psit, 20).backward()  # warm-up
+    amp = torch.tensor(amp0, requires_grad=True)
+    t0 = perf_counter()
+    L = A.overlap_loss(A.gaussian_bumps(amp, centres, 1.5, shape), eng, psi0, psit, T)
+    t1 = perf_counter()
+    L.backward()
+    t2 = perf_counter()
+    grad = amp.grad.numpy().copy()
+    scale = float(np.max(np.abs(grad)))
+    h = 1e-4
+    fd = []
+    for j in fd_bumps:
+        ap, am = amp0.copy(), amp0.copy()
+        ap[j] += h
+        am[j] -= h
+        d = (loss128(ap) - loss128(am)) / (2 * h)
+        fd.append({"bump": j, "fd_c128": d, "adjoint_c64": float(grad[j]), "abs_err_over_max_grad": abs(d - grad[j]) / scale})
+    g.close()
+    # the oracle's FP64 adjoint on the first oracle_steps steps against the engine's
+    from oracle import pyoracle
+
+    V = A.gaussian_bumps(torch.tensor(amp0), centres, 1.5, shape).numpy()
+    o = pyoracle.OracleEngine(shape, 1.0, 0.0, dt_a, float(n), 0, 3, 0)
+    o.set_vext_array(V)
+    o.set_psi(psi0)
+    t3 = perf_counter()
+    Lo, go = o.adjoint_overlap(oracle_steps, psit)
+    t_or = perf_counter() - t3
+    o.close()
+    eng.set_external_potential(V)
+    eng.set_psi(psi0)
+    Lg = eng.adjoint_forward(oracle_steps, psit)
+    gg = eng.adjoint_backward()
+    eng.close()
+    return {
+        "workload": "64^3 complex64 unitary Fermi gas, T=%d real-time steps dt=1e-3, L = 1-|<psi_t|psi(T)>|^2, "
+        "dL/dA for 512 Gaussian bumps of V_ext (A ~ N(0,0.05), width 1.5), sqrt(T) checkpoints" % T,
+        "loss": float(L.item()),
+        "gradient_wall_s": t2 - t0,
+        "forward_s": t1 - t0,
+        "backward_s": t2 - t1,
+        "fd_check": fd,
+        "max_abs_err_vs_fd_over_max_grad": max(x["abs_err_over_max_grad"] for x in fd),
+        "oracle_sample": {
+            "steps": oracle_steps,
+            "loss_abs_err": abs(Lg - Lo),
+            "dLdV_rel_l2_err": float(np.linalg.norm(gg - go) / np.linalg.norm(go)),
+            "oracle_s": t_or,
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +305,9 @@ def main():
     ap.add_argument("--no-c128", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--adjoint-oracle-steps", type=int, default=200)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this engine runs on one GPU; a %d-GPU run is not part of this build "
@@ -317,6 +410,10 @@ def main():
     if not a.no_ensemble:
         print("[bench] batched ensemble (%d steps)" % a.ensemble_steps, file=sys.stderr, flush=True)
         result["ensemble"] = ensemble_bench(E, a.ensemble_steps, 50, 8, 100)
+
+    if not a.no_adjoint:
+        print("[bench] adjoint gradient (T=%d)" % a.adjoint_steps, file=sys.stderr, flush=True)
+        result["adjoint"] = adjoint_bench(E, a.adjoint_steps, a.adjoint_oracle_steps)
 
     if not a.no_cpu_baseline:
         print("[bench] FP64 CPU oracle baseline (%d steps)" % a.cpu_steps, file=sys.stderr, flush=True)

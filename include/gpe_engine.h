@@ -117,6 +117,16 @@ int gpe_evolve_vortex2(int nt, double *chemical_potential GPE_DEFAULT(NULL));
 int gpe_evolve_vortex3(int nt, double *chemical_potential GPE_DEFAULT(NULL));
 /* dev/gpe_engine.h:494: phase from gpe_set_phase / gpe_get_phase enforced */
 int gpe_evolve_enforced_phase(int nt, double *chemical_potential GPE_DEFAULT(NULL));
+/* dev/gpe_engine.h:504, 514: dipolar bosons (INTERACTIONS -2), Strang-Euler
+ * step with the convolution potential F^-1[Vpp(k) F[rho]]/N.  The reference's
+ * (int, double*) overload is gpe_evolve_dipolar_mu at the C level; C++
+ * callers get the overload below. */
+int gpe_evolve_dipolar(int nt);
+int gpe_evolve_dipolar_mu(int nt, double *chemical_potential);
+/* dev/gpe_engine.h:524, 534: dipolar energy in k space (sum_k Vpp |rho_k|^2 / N)
+ * and in real space ((1/2) sum rho V_dip) */
+int gpe_energy_dipolar(double *t, double *edip);
+int gpe_energy_dipolar2(double *t, double *edip);
 /* dev/gpe_engine.h:545 */
 int gpe_energy(double *t, double *ekin, double *eint, double *eext);
 /* dev/gpe_engine.h:552 */
@@ -179,6 +189,9 @@ int gpe_engine_get_currents(gpe_engine_t e, double *t, double *jx, double *jy, d
 int gpe_engine_get_phase(gpe_engine_t e, double *h_phase);
 int gpe_engine_set_phase(gpe_engine_t e, const double *h_phase);
 int gpe_engine_detect_vortices(gpe_engine_t e, double rho_min, int8_t *windings, long *count);
+int gpe_engine_evolve_dipolar(gpe_engine_t e, int nt, double *chemical_potential);
+/* method 1: gpe_energy_dipolar, 2: gpe_energy_dipolar2 */
+int gpe_engine_energy_dipolar(gpe_engine_t e, int method, double *t, double *edip);
 int gpe_engine_get_time(gpe_engine_t e, double *t);
 /* Unnormalised 3-D DFT of psi in place, sign -1 forward / +1 inverse, by the
  * engine's own FFT passes (no counterpart in the reference, whose transform
@@ -284,6 +297,14 @@ int gpe_ensemble_mean_density(gpe_ensemble_t s, int group, double *out);
             exit(EXIT_FAILURE);                                                                  \
         }                                                                                        \
     }
+
+#ifdef __cplusplus
+/* the reference's C++ overload gpe_evolve_dipolar(int, double*) (dev/gpe_engine.h:514) */
+static inline int gpe_evolve_dipolar(int nt, double *chemical_potential)
+{
+    return gpe_evolve_dipolar_mu(nt, chemical_potential);
+}
+#endif
 
 /* A driver compiled with -DNX=.. -DNY=.. -DNZ=.. (the reference's build flags)
  * registers its lattice and physics before main(), so the unchanged

@@ -1056,6 +1056,13 @@ int gpe_detect_vortices(double rho_min, int8_t* w, long* count) {
     return gpe_engine_detect_vortices(g_eng, rho_min, w, count);
 }
 
+int gpe_evolve_dipolar(int nt) { return gpe_engine_evolve_dipolar(g_eng, nt, nullptr); }
+int gpe_evolve_dipolar_mu(int nt, double* chemical_potential) {
+    return gpe_engine_evolve_dipolar(g_eng, nt, chemical_potential);
+}
+int gpe_energy_dipolar(double* t, double* edip) { return gpe_engine_energy_dipolar(g_eng, 1, t, edip); }
+int gpe_energy_dipolar2(double* t, double* edip) { return gpe_engine_energy_dipolar(g_eng, 2, t, edip); }
+
 int gpe_set_device(int device) {
     HIP_TRY(hipSetDevice(device));
     return 0;
@@ -1076,6 +1083,100 @@ void gpe_print_potential_type(void) {
     static const char* names[] = {"HARMONIC POTENTIAL", "WOODS-SAXON TUBE", "COSINE LATTICE", "USER ARRAY"};
     int v = g_eng ? g_eng->vext : g_cfg.cfg.vext;
     printf("%s\n", (v >= 0 && v <= 3) ? names[v] : "UNKNOWN");
+}
+
+}  // extern "C"
+
+// ====================================================================== dipolar
+//
+// INTERACTIONS -2 (dev/gpe_engine.cu:1490-1605): per step two potential half
+// steps around the kinetic step, each with the convolution potential
+// V_dip = Re F^-1[Vpp(k) F[rho]]/N of the current psi (the density goes to B
+// as a complex array, one forward and one inverse FFT with the Vpp factor at
+// the turn-around), then the ITE renormalisation.
+
+namespace {
+
+template <typename T>
+void dipolar_potential_t(gpe_engine_s* e) {
+    launch_density_c<T>((const cplx<T>*)e->A, (cplx<T>*)e->B, e->gamma, e->n, e->stream);
+    PassArgs<T> a = base_args<T>(e);
+    void* B = e->B;
+    run_pass<T>(e, a, {2, B, B, -1, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+    run_pass<T>(e, a, {1, B, B, -1, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+    run_pass<T>(e, a, {0, B, B, -1, 1, LOAD_PLAIN, MID_VPP, STORE_PLAIN, 0, 0});
+    run_pass<T>(e, a, {1, B, B, 1, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+    run_pass<T>(e, a, {2, B, B, 1, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+}
+
+template <typename T>
+void kinetic_apply(gpe_engine_s* e, void* buf, bool adjoint);
+
+template <typename T>
+int evolve_dipolar_t(gpe_engine_s* e, int nt, double* mu) {
+    const bool ite = (e->beta != 0.0);
+    for (int s = 0; s < nt; s++) {
+        const Phys<T> ph = base_args<T>(e).ph;
+        dipolar_potential_t<T>(e);
+        launch_dipolar_vstep<T>((cplx<T>*)e->A, (const cplx<T>*)e->B, ph, e->n, e->stream);
+        kinetic_apply<T>(e, e->A, false);
+        dipolar_potential_t<T>(e);
+        launch_dipolar_vstep<T>((cplx<T>*)e->A, (const cplx<T>*)e->B, ph, e->n, e->stream);
+        if (ite) {
+            int r = normalize_t<T>(e);
+            if (r) return r;
+        }
+        e->it = e->it + 1;
+    }
+    HIP_TRY(hipGetLastError());
+    if (ite && mu) {
+        HIP_TRY(hipMemcpyAsync(mu, e->scal + 2, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+// method 1: sum_k Vpp(k) |rho_k|^2 / N (dev 1716-1760, -k taken modulo the
+// lattice; for the real density Re(rho(-k) Vpp rho(k)) = Vpp |rho(k)|^2);
+// method 2: (1/2) sum rho V_dip (dev 1768-1810)
+template <typename T>
+int energy_dipolar_t(gpe_engine_s* e, int method, double* t, double* edip) {
+    *t = e->t0 + e->dt * e->it;
+    if (method == 1) {
+        launch_density_c<T>((const cplx<T>*)e->A, (cplx<T>*)e->B, e->gamma, e->n, e->stream);
+        PassArgs<T> a = base_args<T>(e);
+        void* B = e->B;
+        run_pass<T>(e, a, {2, B, B, -1, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+        run_pass<T>(e, a, {1, B, B, -1, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+        const int nb = run_pass<T>(e, a, {0, B, nullptr, -1, 0, LOAD_PLAIN, MID_VPP_SUM, STORE_NONE, 0, 1});
+        finalize(e, nb, 1, FIN_STORE, kSlotPot, (double)e->n);
+    } else {
+        dipolar_potential_t<T>(e);
+        const int nb = launch_dipolar_e2<T>((const cplx<T>*)e->A, (const cplx<T>*)e->B, e->gamma, e->n,
+                                            e->partials, e->stream);
+        finalize(e, nb, 1, FIN_STORE, kSlotPot, 2.0);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(edip, e->scal + kSlotPot, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpe_engine_evolve_dipolar(gpe_engine_t e, int nt, double* chemical_potential) {
+    if (!e) return GPE_ERR_NO_ENGINE;
+    if (e->interactions != INT_DIPOLAR || e->batch != 1 || nt < 0) return GPE_ERR_ARG;
+    return DISPATCH(e, evolve_dipolar_t, e, nt, chemical_potential);
+}
+
+int gpe_engine_energy_dipolar(gpe_engine_t e, int method, double* t, double* edip) {
+    if (!e) return GPE_ERR_NO_ENGINE;
+    if (e->interactions != INT_DIPOLAR || e->batch != 1 || (method != 1 && method != 2) || !t || !edip)
+        return GPE_ERR_ARG;
+    return DISPATCH(e, energy_dipolar_t, e, method, t, edip);
 }
 
 }  // extern "C"

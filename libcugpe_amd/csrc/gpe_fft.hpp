@@ -301,7 +301,19 @@ enum MidMode {
     MID_KA = 8,          // k_a / (gamma N) (orginal 1125-1162)
     MID_OVERLAP_RE = 9,  // Re(conj psi * v) -> real output (orginal 1067-1074)
     MID_EXPTC = 10,      // conj(Kx Ky Kz)/N: the adjoint of the kinetic factor
+    MID_VPP = 11,        // Vpp(k)/N: dipolar convolution kernel (dev/gpe_engine.cu:960-975)
+    MID_VPP_SUM = 12,    // sum Vpp(k) |rho_k|^2 (dev/gpe_engine.cu:1716-1735)
 };
+
+// Fourier transform of the particle-particle potential of z-polarised dipoles
+// with contact part, gpe_Vpp_k (dev/gpe_user_defined.h:422-438):
+// a_con = up[3], a_dip = up[4]
+__device__ __forceinline__ double vpp_k(const double* up, double kx, double ky, double kz) {
+    const double PI = 3.14159265358979323846;
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    if (k2 < 2.220446049250313e-16) return 4. * PI * up[3];
+    return 4. * PI * (up[3] + up[4] * (3. * kz * kz / k2 - 1.));
+}
 enum StoreMode { STORE_PLAIN = 0, STORE_NONE = 1 };
 enum PassFlags {
     F_SCALE = 1,          // multiply loaded psi0 / phi3 by the device normalisation factor
@@ -445,6 +457,11 @@ __device__ __forceinline__ cplx<T> mid_op_lean(const PassArgs<T>& a, const Coord
         const T k2 = a.kx[c.ix] * a.kx[c.ix] + a.ky[c.iy] * a.ky[c.iy] + a.kz[c.iz] * a.kz[c.iz];
         return cscale(v, a.kfac * k2);
     }
+    case MID_VPP: {
+        const double f = vpp_k(a.up64, a.kx64[c.ix], a.ky64[c.iy], a.kz64[c.iz]) /
+                         ((double)a.ph.nx * (double)a.ph.ny * (double)a.ph.nz);
+        return mk((T)((double)v.x * f), (T)((double)v.y * f));
+    }
     case MID_KA: {
         const T k = (a.ka_axis == 0) ? a.kx[c.ix] : (a.ka_axis == 1) ? a.ky[c.iy] : a.kz[c.iz];
         return cscale(v, k / a.kfac);
@@ -564,11 +581,15 @@ __device__ __forceinline__ cplx<T> mid_op_m(const PassArgs<T>& a, const ElemIn<T
                           a.kz64[e.c.iz] * a.kz64[e.c.iz];
         acc[0] += k2 * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
         return v;
+    } else if constexpr (MM == MID_VPP_SUM) {
+        const double f = vpp_k(a.up64, a.kx64[e.c.ix], a.ky64[e.c.iy], a.kz64[e.c.iz]);
+        acc[0] += f * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+        return v;
     } else if constexpr (MM == MID_OVERLAP_RE) {
         const cplx<T> p = e.aux;
         a.rout[e.c.off] = (double)(p.x * v.x + p.y * v.y);
         return v;
-    } else if constexpr (MM == MID_EXPT || MM == MID_EXPTC || MM == MID_K2 || MM == MID_KA) {
+    } else if constexpr (MM == MID_EXPT || MM == MID_EXPTC || MM == MID_K2 || MM == MID_KA || MM == MID_VPP) {
         return mid_op_lean(a, e.c, v);
     } else {
         return v;
@@ -583,11 +604,13 @@ __device__ __forceinline__ cplx<T> mid_op2(const PassArgs<T>& a, const ElemIn<T>
     case MID_VSTEP3: return mid_op_m<MID_VSTEP3>(a, e, v, acc);
     case MID_QF_VSTEP1: return mid_op_m<MID_QF_VSTEP1>(a, e, v, acc);
     case MID_KIN_SUM: return mid_op_m<MID_KIN_SUM>(a, e, v, acc);
+    case MID_VPP_SUM: return mid_op_m<MID_VPP_SUM>(a, e, v, acc);
     case MID_OVERLAP_RE: return mid_op_m<MID_OVERLAP_RE>(a, e, v, acc);
     case MID_EXPT:
     case MID_EXPTC:
     case MID_K2:
-    case MID_KA: return mid_op_lean(a, e.c, v);
+    case MID_KA:
+    case MID_VPP: return mid_op_lean(a, e.c, v);
     default: return v;
     }
 }
@@ -1122,7 +1145,7 @@ template <typename T>
 inline bool pass_is_lean(const PassArgs<T>& a) {
     return a.load_mode == LOAD_PLAIN && a.flags == 0 && a.nred == 0 &&
            (a.mid_mode == MID_NONE || a.mid_mode == MID_EXPT || a.mid_mode == MID_EXPTC || a.mid_mode == MID_K2 ||
-            a.mid_mode == MID_KA);
+            a.mid_mode == MID_KA || a.mid_mode == MID_VPP);
 }
 
 inline int pass_shape(int dir1, int dir2) {

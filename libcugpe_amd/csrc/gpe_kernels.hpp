@@ -76,6 +76,15 @@ void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 constexpr int kMaxReduceBlocks = 1024;
 
+// dipolar path (gpe_dipolar.hip)
+template <typename T>
+void launch_density_c(const cplx<T>* A, cplx<T>* B, double gamma, size_t n, hipStream_t s);
+template <typename T>
+void launch_dipolar_vstep(cplx<T>* A, const cplx<T>* B, const Phys<T>& ph, size_t n, hipStream_t s);
+// partials of sum gamma |A|^2 Re B; nb returned
+template <typename T>
+int launch_dipolar_e2(const cplx<T>* A, const cplx<T>* B, double gamma, size_t n, double* partials, hipStream_t s);
+
 // discrete adjoint of the real-time split step (gpe_adjoint.hip)
 struct AdjArgs {
     int interactions;

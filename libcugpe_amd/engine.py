@@ -81,6 +81,8 @@ def lib():
             "gpe_engine_profile_read": [_p, _dp, ctypes.POINTER(ctypes.c_long)],
             "gpe_engine_copy_benchmark": [_p, _dp],
             "gpe_engine_autotune": [_p, _i, ctypes.POINTER(_i)],
+            "gpe_engine_evolve_dipolar": [_p, _i, _dp],
+            "gpe_engine_energy_dipolar": [_p, _i, _dp, _dp],
             "gpe_engine_adjoint_forward": [_p, _i, _p, _i, _dp],
             "gpe_engine_adjoint_backward": [_p, _p],
             "gpe_engine_set_variant": [_p, _i],
@@ -293,6 +295,18 @@ class Engine:
         ms = ctypes.c_double()
         _check(lib().gpe_engine_copy_benchmark(self._h, ctypes.byref(ms)), "gpe_engine_copy_benchmark")
         return ms.value
+
+    def evolve_dipolar(self, nt, want_mu=False):
+        mu = ctypes.c_double(float("nan"))
+        _check(lib().gpe_engine_evolve_dipolar(self._h, int(nt), ctypes.byref(mu) if want_mu else None),
+               "gpe_evolve_dipolar")
+        return mu.value if want_mu else None
+
+    def energy_dipolar(self, method=1):
+        t, ed = ctypes.c_double(), ctypes.c_double()
+        _check(lib().gpe_engine_energy_dipolar(self._h, int(method), ctypes.byref(t), ctypes.byref(ed)),
+               "gpe_energy_dipolar")
+        return t.value, ed.value
 
     def adjoint_forward(self, nt, psi_target, ckpt=0):
         """nt real-time steps keeping checkpoints; returns

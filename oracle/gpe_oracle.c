@@ -1006,3 +1006,140 @@ int gpeo_adjoint_overlap(gpeo_t *e, int nt, const double *psi_target, int ckpt, 
     free(final_psi); free(ck); free(seg); free(lam); free(phi); free(lam0); free(vb); free(pl);
     return 0;
 }
+
+/* ------------------------------------------------------------------ dipolar
+ *
+ * INTERACTIONS -2: bosons with contact and z-polarised dipole-dipole
+ * interactions (dev/gpe_user_defined.h:356-438).  The mean field is the
+ * convolution V_dip = Re F^-1[ Vpp(k) F[rho] ] / N with
+ *   Vpp(k) = 4 pi a_con                                   for |k|^2 < DBL_EPSILON,
+ *            4 pi (a_con + a_dip (3 kz^2 / |k|^2 - 1))   otherwise
+ * (gpe_Vpp_k, dev/gpe_user_defined.h:422-438; a_con = up[3], a_dip = up[4]);
+ * the local dEDF/dn is 0 and the step is the Strang-Euler one of
+ * gpe_evolve_dipolar (dev/gpe_engine.cu:1490-1605).
+ */
+static double vpp_k(const gpeo_t *e, double kx, double ky, double kz)
+{
+    const double ac = e->up[3], ad = e->up[4];
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    if (k2 < DBL_EPSILON) return 4. * M_PI * ac;
+    return 4. * M_PI * (ac + ad * (3. * kz * kz / k2 - 1.));
+}
+
+/* wrk = (rho, 0) -> F -> x Vpp/N -> F^-1: __gpe_compute_density2C__ (dev 618-626),
+ * __gpe_compute_vint_k__ (dev 960-975) between the two transforms */
+static void dipolar_potential(gpeo_t *e, const cd *psi, cd *wrk)
+{
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < e->n; i++) {
+        wrk[i].re = density(e, psi[i]);
+        wrk[i].im = 0.0;
+    }
+    fft3(e, wrk, -1);
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++)
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                wrk[i] = cscale(wrk[i], vpp_k(e, e->kx[ix], e->ky[iy], e->kz[iz]) / (double)e->n);
+            }
+    fft3(e, wrk, +1);
+}
+
+/* __gpe_eulerstrang_exp_Vstep__: dev/gpe_engine.cu:984-1008 */
+static void eulerstrang_vstep(gpeo_t *e, cd *psi, const cd *vdip)
+{
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++)
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                cd l = modify_psi(psi[i]);
+                double r = density(e, l);
+                double lv = vext_val(e, ix, iy, iz, i) + dedfdn(e, r) + vdip[i].re;
+                psi[i] = cmul(l, cexp_(cscale(e->c, lv)));
+            }
+}
+
+/* gpe_evolve_dipolar(int) / (int, double*): dev/gpe_engine.cu:1490-1605 */
+int gpeo_evolve_dipolar(gpeo_t *e, int nt, double *mu)
+{
+    for (int s = 0; s < nt; s++) {
+        dipolar_potential(e, e->psi, e->wrk);
+        eulerstrang_vstep(e, e->psi, e->wrk);
+        fft3(e, e->psi, -1);
+        multiply_by_expT(e, e->psi);
+        fft3(e, e->psi, +1);
+        dipolar_potential(e, e->psi, e->wrk);
+        eulerstrang_vstep(e, e->psi, e->wrk);
+        if (e->beta != 0.0) {
+            double norm = normalize(e, e->psi);
+            if (mu) *mu = -.5 * log(norm / e->npart) / e->dt;
+        }
+        e->it = e->it + 1;
+    }
+    return 0;
+}
+
+/* gpe_energy_dipolar: dev/gpe_engine.cu:1716-1760.  sum_k Re(rho_k(-k) Vpp(k)
+ * rho_k(k)) / N, with -k taken modulo the lattice (the reference indexes
+ * n - i, which leaves the lattice for i = 0). */
+int gpeo_energy_dipolar(gpeo_t *e, double *t, double *edip)
+{
+    *t = e->t0 + e->dt * e->it;
+    cd *rk = e->wrk;
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < e->n; i++) {
+        rk[i].re = density(e, e->psi[i]);
+        rk[i].im = 0.0;
+    }
+    fft3(e, rk, -1);
+    double *pl = plane_buf(e);
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++) {
+        double s = 0.0;
+        const int rx = (e->nx - ix) % e->nx;
+        for (int iy = 0; iy < e->ny; iy++) {
+            const int ry = (e->ny - iy) % e->ny;
+            for (int iz = 0; iz < e->nz; iz++) {
+                const int rz = (e->nz - iz) % e->nz;
+                size_t i = IDX3(ix, iy, iz), j = IDX3(rx, ry, rz);
+                cd en = cscale(rk[i], vpp_k(e, e->kx[ix], e->ky[iy], e->kz[iz]));
+                s += rk[j].re * en.re - rk[j].im * en.im;
+            }
+        }
+        pl[ix] = s;
+    }
+    *edip = ordered_sum_planes(pl, e->nx) / (double)e->n;
+    free(pl);
+    return 0;
+}
+
+/* gpe_energy_dipolar2: dev/gpe_engine.cu:1768-1810.  (1/2) sum rho Re V_dip */
+int gpeo_energy_dipolar2(gpeo_t *e, double *t, double *edip)
+{
+    *t = e->t0 + e->dt * e->it;
+    dipolar_potential(e, e->psi, e->wrk);
+    double *pl = plane_buf(e);
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < e->nx; ix++) {
+        double s = 0.0;
+        for (int iy = 0; iy < e->ny; iy++)
+            for (int iz = 0; iz < e->nz; iz++) {
+                size_t i = IDX3(ix, iy, iz);
+                s += density(e, e->psi[i]) * e->wrk[i].re;
+            }
+        pl[ix] = s;
+    }
+    *edip = 0.5 * ordered_sum_planes(pl, e->nx);
+    free(pl);
+    return 0;
+}
+
+/* the convolution potential alone (for the pinning tests) */
+int gpeo_dipolar_potential(gpeo_t *e, double *vdip)
+{
+    dipolar_potential(e, e->psi, e->wrk);
+    for (size_t i = 0; i < e->n; i++) vdip[i] = e->wrk[i].re;
+    return 0;
+}

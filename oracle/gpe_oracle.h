@@ -96,6 +96,15 @@ long gpeo_plaquette_windings(int nx, int ny, int nz, const double *psi64,
  * (otherwise -12).  Leaves psi at psi(T). */
 int gpeo_adjoint_overlap(gpeo_t *e, int nt, const double *psi_target, int ckpt, double *loss, double *grad);
 
+/* Dipolar bosons (INTERACTIONS -2): the Strang-Euler step of
+ * gpe_evolve_dipolar, the k-space and real-space dipolar energies of
+ * gpe_energy_dipolar / gpe_energy_dipolar2, and the convolution potential
+ * V_dip = Re F^-1[Vpp(k) F[rho]]/N on its own (a_con = up[3], a_dip = up[4]). */
+int gpeo_evolve_dipolar(gpeo_t *e, int nt, double *mu);
+int gpeo_energy_dipolar(gpeo_t *e, double *t, double *edip);
+int gpeo_energy_dipolar2(gpeo_t *e, double *t, double *edip);
+int gpeo_dipolar_potential(gpeo_t *e, double *vdip);
+
 int gpeo_num_threads(void);
 
 #ifdef __cplusplus

@@ -67,6 +67,10 @@ def lib():
             ("gpeo_fft3d", [_i, _i, _i, _p, _i]),
             ("gpeo_fft1d", [_i, _p, _i]),
             ("gpeo_adjoint_overlap", [_p, _i, _p, _i, _dp, _p]),
+            ("gpeo_evolve_dipolar", [_p, _i, _dp]),
+            ("gpeo_energy_dipolar", [_p, _dp, _dp]),
+            ("gpeo_energy_dipolar2", [_p, _dp, _dp]),
+            ("gpeo_dipolar_potential", [_p, _p]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -231,6 +235,22 @@ class OracleEngine:
         t = ctypes.c_double()
         assert lib().gpeo_get_density(self._h, ctypes.byref(t), _ptr(a)) == 0
         return a, t.value
+
+    def evolve_dipolar(self, nt, want_mu=False):
+        mu = ctypes.c_double(float("nan"))
+        assert lib().gpeo_evolve_dipolar(self._h, int(nt), ctypes.byref(mu) if want_mu else None) == 0
+        return mu.value if want_mu else None
+
+    def energy_dipolar(self, method=1):
+        t, ed = ctypes.c_double(), ctypes.c_double()
+        fn = lib().gpeo_energy_dipolar if method == 1 else lib().gpeo_energy_dipolar2
+        assert fn(self._h, ctypes.byref(t), ctypes.byref(ed)) == 0
+        return t.value, ed.value
+
+    def dipolar_potential(self):
+        v = np.empty(self.shape, dtype=np.float64)
+        assert lib().gpeo_dipolar_potential(self._h, _ptr(v)) == 0
+        return v
 
     def adjoint_overlap(self, nt, psi_target, ckpt=0):
         """(L, dL/dV_ext) of L = 1 - |<psi_t|psi(T)>|^2/(|psi_t|^2 |psi(0)|^2) over nt steps."""

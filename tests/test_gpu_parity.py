@@ -292,3 +292,47 @@ def test_launcher_variants_match_oracle(oracle, variant, shape):
         if mode == "ite":
             assert abs(mg - mo) < TOL128 * abs(mo)
         g.close()
+
+
+@pytest.mark.parametrize("mode", ["rte", "ite"])
+@pytest.mark.parametrize("shape", [(32, 24, 16), (24, 16, 12)])
+def test_dipolar_matches_oracle(oracle, mode, shape):
+    """gpe_evolve_dipolar and both dipolar energies (dev/gpe_engine.cu:1490-1810)."""
+    say("dipolar", mode, shape)
+    alpha, beta = (1.0, 0.0) if mode == "rte" else (0.0, 1.0)
+    params = [0.05, 0.06, 0.08, 0.03, 0.02]  # trap, a_con, a_dip
+    g, o = pair(oracle, shape, alpha, beta, 0.02, 400.0, E.DIPOLAR, E.HARMONIC, params)
+    psi = seeded_state(shape, 31)
+    for x in (g, o):
+        x.set_psi(psi)
+        x.normalize_psi()
+    for nsteps in (1, 6):
+        mg = g.evolve_dipolar(nsteps, want_mu=True)
+        mo = o.evolve_dipolar(nsteps, want_mu=True)
+        assert rel_err(g.get_psi()[0], o.get_psi()[0]) < TOL128
+        if mode == "ite":
+            assert abs(mg - mo) < TOL128 * abs(mo)
+    for method in (1, 2):
+        tg, eg = g.energy_dipolar(method)
+        to, eo = o.energy_dipolar(method)
+        assert tg == to
+        assert abs(eg - eo) < TOL128 * abs(eo)
+    g.close()
+
+
+def test_dipolar_complex64_tracks_oracle(oracle):
+    say("dipolar c64")
+    shape = (32, 32, 32)
+    params = [0.05, 0.05, 0.05, 0.03, 0.02]
+    g, o = pair(oracle, shape, 1.0, 0.0, 0.02, 1000.0, E.DIPOLAR, E.HARMONIC, params, precision=E.C64)
+    psi = seeded_state(shape, 32)
+    for x in (g, o):
+        x.set_psi(psi)
+        x.normalize_psi()
+    g.evolve_dipolar(50)
+    o.evolve_dipolar(50)
+    assert rel_err(g.get_psi()[0], o.get_psi()[0]) < 2e-5
+    _, eg = g.energy_dipolar(2)
+    _, eo = o.energy_dipolar(2)
+    assert abs(eg - eo) < 1e-5 * abs(eo)
+    g.close()
