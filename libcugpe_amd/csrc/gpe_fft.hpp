@@ -386,9 +386,13 @@ __device__ __forceinline__ void sincos_small(float x, float* s, float* c) {
 
 // v * exp(re + i im).  In complex64 the rounded direction (c, s) misses unit
 // modulus by e = c^2 + s^2 - 1 ~ 1e-8, the same way every step; e is formed
-// without cancellation by two FMAs and the product is scaled by 1 - e/2
-// (= 1/|(c, s)| to O(e^2)), so the factor is a rotation to ~1e-15 and only
-// unbiased final roundings remain.
+// without cancellation by two FMAs and the product is scaled by 1 + h,
+// h = -e/2 (= 1/|(c, s)| - 1 to O(e^2)).  The scaling must enter before the
+// product's one significant rounding: |h| is below half an ulp, so applied to
+// an already rounded product it would be rounded away every time.  Hence
+//   rx = fma(vx, c, fma(-vy, s, h vx)),  ry = fma(vy, c, fma(vx, s, h vy)),
+// whose inner terms are small (s ~ |cV|) and whose dropped part h (r - v) is
+// O(h |cV|).  The factor is then a rotation to ~1e-15 up to unbiased roundings.
 __device__ __forceinline__ cplx<double> mul_exp(cplx<double> v, double re, double im) {
     return cmul(v, cexp_(re, im));
 }
@@ -396,10 +400,8 @@ __device__ __forceinline__ cplx<float> mul_exp(cplx<float> v, float re, float im
     float s, c;
     sincos_small(im, &s, &c);
     const float h = -0.5f * fmaf(s, s, fmaf(c, c, -1.0f));
-    float rx = fmaf(v.x, c, -(v.y * s));
-    float ry = fmaf(v.x, s, v.y * c);
-    rx = fmaf(h, rx, rx);
-    ry = fmaf(h, ry, ry);
+    float rx = fmaf(v.x, c, fmaf(-v.y, s, h * v.x));
+    float ry = fmaf(v.y, c, fmaf(v.x, s, h * v.y));
     if (re != 0.0f) {  // real time: re == 0 for every point
         const float g = expf(re);
         rx *= g;

@@ -79,7 +79,11 @@ def test_adjoint_c64_against_c128():
 def test_torch_wrapper_matches_finite_differences():
     """dL/dA for Gaussian-bump amplitudes through autograd (torch ops for the
     bumps, the engine's adjoint for dL/dV) against central differences of the
-    complex128 engine with h = 1e-4."""
+    complex128 engine with h = 1e-4.  The complex128 adjoint must meet them to
+    1e-6 of the largest component.  The complex64 adjoint differentiates the
+    complex64 trajectory, whose overlap (L ~ 1e-3 here) carries absolute
+    errors ~1e-6 from FP32 rounding over the run; its bound is 5e-4 of the
+    largest component (about 2e-8 absolute here)."""
     import torch
 
     from libcugpe_amd import adjoint as A
@@ -122,8 +126,8 @@ def test_torch_wrapper_matches_finite_differences():
         am[j] -= h
         fd = (loss(ap) - loss(am)) / (2 * h)
         say("bump %d: fd %.9e adj128 %.9e adj64 %.9e" % (j, fd, grads[E.C128][j], grads[E.C64][j]))
-        assert abs(fd - grads[E.C128][j]) < 1e-5 * scale
-        assert abs(fd - grads[E.C64][j]) < 1e-5 * scale
+        assert abs(fd - grads[E.C128][j]) < 1e-6 * scale
+        assert abs(fd - grads[E.C64][j]) < 5e-4 * scale
     eng.close()
 
 
