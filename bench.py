@@ -78,6 +78,34 @@ def pmc_traffic():
     return None
 
 
+def rounding_energy_error(E, shape, psi, segments=10, seg_steps=100):
+    """Energy error that complex64 rounding adds over segments*seg_steps steps,
+    free of trajectory divergence: at the start of every segment the complex128
+    engine takes the complex64 engine's state, both run seg_steps, and the
+    difference of their energy changes is summed."""
+    e64 = make_engine(E, shape, E.C64, psi)
+    e128 = make_engine(E, shape, E.C128, psi)
+    total, worst, e_ref = 0.0, 0.0, None
+    for _ in range(segments):
+        cur, _ = e64.get_psi()
+        e128.set_psi(cur)
+        _, a1, a2, a3 = e64.energy()
+        _, b1, b2, b3 = e128.energy()
+        e64.evolve_qf(seg_steps)
+        e128.evolve_qf(seg_steps)
+        _, c1, c2, c3 = e64.energy()
+        _, d1, d2, d3 = e128.energy()
+        if e_ref is None:
+            e_ref = b1 + b2 + b3
+        d = ((c1 + c2 + c3) - (a1 + a2 + a3)) - ((d1 + d2 + d3) - (b1 + b2 + b3))
+        total += d
+        worst = max(worst, abs(d))
+    e64.close()
+    e128.close()
+    return {"steps": segments * seg_steps, "segments": segments, "rel_err": abs(total) / abs(e_ref),
+            "worst_segment_rel_err": worst / abs(e_ref)}
+
+
 def cpu_baseline(shape, psi, steps):
     from oracle import pyoracle
 
@@ -324,6 +352,8 @@ def main():
 
     print("[bench] complex64 engine: warmup %d, timed %d" % (a.warmup, a.steps), file=sys.stderr, flush=True)
     eng = make_engine(E, shape, E.C64, psi)
+    _, ek, ei, ee = eng.energy()
+    e64_0 = ek + ei + ee
     eng.evolve_qf(a.warmup)
     t = timed(eng, a.steps)
     sps = a.steps / t
@@ -396,6 +426,8 @@ def main():
     if not a.no_c128:
         print("[bench] complex128 engine on the same steps", file=sys.stderr, flush=True)
         e128 = make_engine(E, shape, E.C128, psi)
+        _, ek, ei, ee = e128.energy()
+        e128_0 = ek + ei + ee
         e128.evolve_qf(a.warmup)
         t128 = timed(e128, a.steps)
         e128.evolve_qf(total_steps - a.warmup - a.steps)
@@ -406,6 +438,13 @@ def main():
         result["c128_step_hbm_frac"] = 22 * 2 * S * (a.steps / t128) / (HBM_PEAK_GBS * 1e9)
         result["energy_rel_err_c64_vs_c128"] = abs(e64 - eref) / abs(eref)
         result["energy_steps_compared"] = total_steps
+        # each precision's own energy drift over the run: the split step's
+        # truncation error follows the (chaotic) trajectory, rounding does not
+        result["energy_drift_c64"] = e64 / e64_0 - 1
+        result["energy_drift_c128"] = eref / e128_0 - 1
+        result["energy_rel_err_c64_vs_c128_step0"] = abs(e64_0 - e128_0) / abs(e128_0)
+        print("[bench] complex64 rounding error of the energy (segmented)", file=sys.stderr, flush=True)
+        result["energy_rounding_err_c64"] = rounding_energy_error(E, shape, psi)
 
     if not a.no_ensemble:
         print("[bench] batched ensemble (%d steps)" % a.ensemble_steps, file=sys.stderr, flush=True)

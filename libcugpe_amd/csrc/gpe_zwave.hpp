@@ -49,9 +49,6 @@ struct ZW {
 
 // load mode of the energy pass: plain load plus the potential-energy sums
 constexpr int LOAD_REDPOT = 3;
-#ifndef ZW_CHUNK
-#define ZW_CHUNK 2
-#endif
 
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -103,9 +100,80 @@ struct ZwLds {
     double red[kThreads / 64];
 };
 
+// ---- chunk-vectorised physics for the hot real-time ops.  The branches on
+// the physics kind are taken once per chunk, not once per point, and the
+// points of a chunk are independent, so the compiler interleaves them.
+
+template <typename T, int CH>
+__device__ __forceinline__ void dedfdn_chunk(const Phys<T>& p, const cplx<T> (&v)[CH], T (&out)[CH]) {
+    if (p.interactions == INT_CONTACT) {
+        const T k = (T)4 * (T)3.14159265358979323846 * p.up[3] * p.gamma;
+#pragma unroll
+        for (int j = 0; j < CH; j++) out[j] = k * (v[j].x * v[j].x + v[j].y * v[j].y);
+    } else if (p.interactions == INT_DIPOLAR) {
+#pragma unroll
+        for (int j = 0; j < CH; j++) out[j] = (T)0;
+    } else {
+#pragma unroll
+        for (int j = 0; j < CH; j++) out[j] = dedfdn(p, density(p, v[j]));
+    }
+}
+
+// v[j] *= exp(cx lv[j] + i cy lv[j]) (mul_exp per point, the rare large-angle
+// case through sincosf for the lanes that need it)
+template <int CH>
+__device__ __forceinline__ void mul_exp_chunk(cplx<float> (&v)[CH], const float (&lv)[CH], float cx, float cy) {
+    float s[CH], c[CH];
+    bool big = false;
+#pragma unroll
+    for (int j = 0; j < CH; j++) {
+        const float x = cy * lv[j];
+        const float t = x * x;
+        s[j] = fmaf(x * t, fmaf(t, fmaf(t, -1.0f / 5040.0f, 1.0f / 120.0f), -1.0f / 6.0f), x);
+        c[j] = fmaf(t, fmaf(t, fmaf(t, -1.0f / 720.0f, 1.0f / 24.0f), -0.5f), 1.0f);
+        big = big || (fabsf(x) > 0.25f);
+    }
+    if (big) {
+#pragma unroll
+        for (int j = 0; j < CH; j++) {
+            const float x = cy * lv[j];
+            if (fabsf(x) > 0.25f) sincosf(x, &s[j], &c[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < CH; j++) {
+        const float h = -0.5f * fmaf(s[j], s[j], fmaf(c[j], c[j], -1.0f));
+        const float rx = fmaf(v[j].x, c[j], fmaf(-v[j].y, s[j], h * v[j].x));
+        const float ry = fmaf(v[j].y, c[j], fmaf(v[j].x, s[j], h * v[j].y));
+        v[j] = mk(rx, ry);
+    }
+    if (cx != 0.0f) {
+#pragma unroll
+        for (int j = 0; j < CH; j++) {
+            const float gm = expf(cx * lv[j]);
+            v[j] = mk(v[j].x * gm, v[j].y * gm);
+        }
+    }
+}
+template <int CH>
+__device__ __forceinline__ void mul_exp_chunk(cplx<double> (&v)[CH], const double (&lv)[CH], double cx, double cy) {
+#pragma unroll
+    for (int j = 0; j < CH; j++) v[j] = mul_exp(v[j], cx * lv[j], cy * lv[j]);
+}
+
 // SHAPE: F, I or IF; LM: load op before the first transform; MM: op after it
+#ifndef ZW_WAVES_PHYS
+#define ZW_WAVES_PHYS 3
+#endif
+// resident waves per SIMD the compiler must allow (VGPR budget 512 / waves):
+// complex64 4, or ZW_WAVES_PHYS for the passes with physics between transforms
+template <typename T, int SHAPE, int MM>
+constexpr int zw_waves() {
+    return sizeof(T) == 4 ? ((SHAPE == SHAPE_IF && MM != MID_NONE) ? ZW_WAVES_PHYS : 4) : 2;
+}
+
 template <typename T, int N, int SHAPE, int LM, int MM>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(zw_waves<T, SHAPE, MM>())))
 void zwave_kernel(const PassArgs<T> a) {
     using Z = ZW<T, N>;
     constexpr int A = Z::A, B = Z::B;
@@ -134,6 +202,7 @@ void zwave_kernel(const PassArgs<T> a) {
     const T vscale = (vk == VEXT_COSINE) ? a.ph.up[0] : (T)1;
     const T vxy = sep ? a.ph.vtab[c.ix] + a.ph.vtab[a.ph.nx + c.iy] : (T)0;
     double acc[3] = {0.0, 0.0, 0.0};
+    auto off_of = [&](int b) { return base + (unsigned)(A * b); };
 
     // the external potential and the coordinates of register b
     auto elem_at = [&](int b) {
@@ -154,59 +223,153 @@ void zwave_kernel(const PassArgs<T> a) {
 #pragma unroll
         for (int b = 0; b < B; b++) v[b] = src[base + (unsigned)(A * b)];
     }
-    if constexpr (LM != LOAD_PLAIN) {
-        // LOAD_REDPOT: the energy sums of rho V_ext and EDF(rho) on the loaded psi
-        constexpr int LMO = (LM == LOAD_REDPOT) ? LOAD_PLAIN : LM;
-        const T* rs = real_side(a);
+
+    // Side inputs (psi0, V1 or V_qf) and the per-point physics run in chunks
+    // of CH registers; the loads of chunk q+1 are issued before chunk q is
+    // computed, and those of chunk 0 before the first transform, so their
+    // latency overlaps arithmetic.
+#ifndef ZW_CH
+#define ZW_CH 4
+#endif
+#ifndef ZW_PREFETCH
+#define ZW_PREFETCH 1
+#endif
+    constexpr int CH = (B >= ZW_CH) ? ZW_CH : B;
+    constexpr int NCH = B / CH;
+    constexpr bool PRE = (LM != LOAD_PLAIN);   // physics before the first transform
+    constexpr bool POST = (MM != MID_NONE);    // physics after it
+    constexpr bool NEED_AUX = (LM == LOAD_PART2) || MM == MID_VSTEP2 || MM == MID_PART1 ||
+                              MM == MID_QF_VSTEP1 || MM == MID_OVERLAP_RE;
+    const T* rs = real_side(a);
+    const bool qf = (a.flags & F_QF) != 0;
+    // the fast chunk forms: real-time ops without imprint
+    const bool fast = (MM == MID_VSTEP2) || (MM == MID_VSTEP3 && !(a.flags & F_IMPRINT_NEXT)) ||
+                      (LM == LOAD_PSI0 && !(a.flags & F_IMPRINT));
+    cplx<T> ax[2][CH];
+    T rx[2][CH];
+    auto fetch = [&](int q, int slot) {
 #pragma unroll
-        for (int b0 = 0; b0 < B; b0 += ZW_CHUNK) {
-            cplx<T> aux[ZW_CHUNK];
-            T raux[ZW_CHUNK];
+        for (int j = 0; j < CH; j++) {
+            const unsigned off = base + (unsigned)(A * (q * CH + j));
+            ax[slot][j] = NEED_AUX ? a.psiA[off] : mk((T)0, (T)0);
+        }
+        if (rs) {
 #pragma unroll
-            for (int q = 0; q < ZW_CHUNK; q++) {
-                const unsigned off = base + (unsigned)(A * (b0 + q));
-                aux[q] = (LM == LOAD_PART2) ? a.psiA[off] : mk((T)0, (T)0);
-                raux[q] = rs ? rs[off] : (T)0;
-            }
-            if (ok) {
+            for (int j = 0; j < CH; j++) rx[slot][j] = rs[off_of(q * CH + j)];
+        } else {
 #pragma unroll
-                for (int q = 0; q < ZW_CHUNK; q++) {
-                    ElemIn<T> e = elem_at(b0 + q);
-                    e.aux = aux[q];
-                    e.raux = raux[q];
-                    v[b0 + q] = load_op_m<LMO>(a, e, v[b0 + q], acc);
-                }
+            for (int j = 0; j < CH; j++) rx[slot][j] = (T)0;
+        }
+    };
+    auto vx_chunk = [&](int q, T (&vx)[CH]) {
+        if (sep) {
+#pragma unroll
+            for (int j = 0; j < CH; j++) vx[j] = vscale * (vxy + L.vz[l + A * (q * CH + j)]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < CH; j++) {
+                const int b = q * CH + j;
+                vx[j] = vext(a.ph, c.ix, c.iy, (unsigned)(l + A * b), base + (unsigned)(A * b));
             }
         }
-    }
+    };
+    // one chunk of physics on v[q CH .. q CH + CH)
+    auto physics = [&](int q, int slot) {
+        if (fast) {
+            T vx[CH], gv[CH], lv[CH];
+            cplx<T> u[CH];
+            vx_chunk(q, vx);
+#pragma unroll
+            for (int j = 0; j < CH; j++) u[j] = v[q * CH + j];
+            if constexpr (LM == LOAD_PSI0) {
+                // rescale, then Vstep1 (orginal/gpe_engine.cu:723-772)
+                if (a.flags & F_SCALE) {
+                    const T f = (T)a.scal[1];
+#pragma unroll
+                    for (int j = 0; j < CH; j++) u[j] = cscale(u[j], f);
+                    if (ok) {
+#pragma unroll
+                        for (int j = 0; j < CH; j++) a.psiA[off_of(q * CH + j)] = u[j];
+                    }
+                }
+                if (a.flags & F_VSTEP1) {
+                    dedfdn_chunk(a.ph, u, gv);
+#pragma unroll
+                    for (int j = 0; j < CH; j++) lv[j] = vx[j] + gv[j] + (qf ? rx[slot][j] : (T)0);
+                    mul_exp_chunk(u, lv, a.ph.c.x, a.ph.c.y);
+                }
+            } else if constexpr (MM == MID_VSTEP2) {
+                // predictor end and corrector potential (orginal/gpe_engine.cu:774-801)
+                cplx<T> p0[CH];
+#pragma unroll
+                for (int j = 0; j < CH; j++) p0[j] = ax[slot][j];
+                T V0[CH];
+                dedfdn_chunk(a.ph, p0, gv);
+#pragma unroll
+                for (int j = 0; j < CH; j++) V0[j] = vx[j] + gv[j] + (qf ? rx[slot][j] : (T)0);
+                mul_exp_chunk(u, V0, a.ph.c.x, a.ph.c.y);  // phi3
+                dedfdn_chunk(a.ph, u, gv);
+#pragma unroll
+                for (int j = 0; j < CH; j++) lv[j] = (T)0.5 * (V0[j] + vx[j] + gv[j] + (qf ? rx[slot][j] : (T)0));
+                if (ok) {
+#pragma unroll
+                    for (int j = 0; j < CH; j++) a.vbuf[off_of(q * CH + j)] = lv[j];
+                }
+                mul_exp_chunk(p0, lv, a.ph.c.x, a.ph.c.y);
+#pragma unroll
+                for (int j = 0; j < CH; j++) u[j] = p0[j];
+            } else if constexpr (MM == MID_VSTEP3) {
+                // corrector end (orginal/gpe_engine.cu:870-878), then the next step's Vstep1
+#pragma unroll
+                for (int j = 0; j < CH; j++) lv[j] = rx[slot][j];
+                mul_exp_chunk(u, lv, a.ph.c.x, a.ph.c.y);  // psi1
+                if ((a.flags & F_RED_NORM) && ok) {
+#pragma unroll
+                    for (int j = 0; j < CH; j++) acc[0] += dens64((double)a.ph.gamma, (double)u[j].x, (double)u[j].y);
+                }
+                if (ok) {
+#pragma unroll
+                    for (int j = 0; j < CH; j++) a.psiA[off_of(q * CH + j)] = u[j];
+                }
+                if (a.flags & F_NEXT_VSTEP1) {
+                    dedfdn_chunk(a.ph, u, gv);
+#pragma unroll
+                    for (int j = 0; j < CH; j++) lv[j] = vx[j] + gv[j];
+                    mul_exp_chunk(u, lv, a.ph.c.x, a.ph.c.y);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < CH; j++) v[q * CH + j] = u[j];
+        } else if (ok) {
+            constexpr int LMO = (LM == LOAD_REDPOT) ? LOAD_PLAIN : LM;
+#pragma unroll
+            for (int j = 0; j < CH; j++) {
+                ElemIn<T> e = elem_at(q * CH + j);
+                e.aux = ax[slot][j];
+                e.raux = rx[slot][j];
+                if constexpr (PRE) v[q * CH + j] = load_op_m<LMO>(a, e, v[q * CH + j], acc);
+                else v[q * CH + j] = mid_op_m<MM>(a, e, v[q * CH + j], acc);
+            }
+        }
+    };
 
     constexpr int S1 = (SHAPE == SHAPE_F || SHAPE == SHAPE_FI) ? -1 : 1;
     constexpr int S2 = (SHAPE == SHAPE_FI) ? 1 : (SHAPE == SHAPE_IF) ? -1 : 0;
+    if constexpr (PRE || (POST && ZW_PREFETCH)) fetch(0, 0);
+    if constexpr (PRE) {
+#pragma unroll
+        for (int q = 0; q < NCH; q++) {
+            if (q + 1 < NCH) fetch(q + 1, (q + 1) & 1);
+            physics(q, q & 1);
+        }
+    }
     zw_fft<T, N, S1, B>(v, X, L.tw, L.twlo, l);
-
-    if constexpr (MM != MID_NONE) {
-        constexpr bool want_psi0 = (MM == MID_VSTEP2 || MM == MID_PART1 || MM == MID_QF_VSTEP1 ||
-                                    MM == MID_OVERLAP_RE);
-        const T* rs = real_side(a);
+    if constexpr (POST && !ZW_PREFETCH) fetch(0, 0);
+    if constexpr (POST) {
 #pragma unroll
-        for (int b0 = 0; b0 < B; b0 += ZW_CHUNK) {
-            cplx<T> aux[ZW_CHUNK];
-            T raux[ZW_CHUNK];
-#pragma unroll
-            for (int q = 0; q < ZW_CHUNK; q++) {
-                const unsigned off = base + (unsigned)(A * (b0 + q));
-                aux[q] = want_psi0 ? a.psiA[off] : mk((T)0, (T)0);
-                raux[q] = rs ? rs[off] : (T)0;
-            }
-            if (ok) {
-#pragma unroll
-                for (int q = 0; q < ZW_CHUNK; q++) {
-                    ElemIn<T> e = elem_at(b0 + q);
-                    e.aux = aux[q];
-                    e.raux = raux[q];
-                    v[b0 + q] = mid_op_m<MM>(a, e, v[b0 + q], acc);
-                }
-            }
+        for (int q = 0; q < NCH; q++) {
+            if (q + 1 < NCH) fetch(q + 1, (q + 1) & 1);
+            physics(q, q & 1);
         }
     }
     if constexpr (S2 != 0) zw_fft<T, N, S2, B>(v, X, L.tw, L.twlo, l);
