@@ -429,47 +429,70 @@ __device__ __forceinline__ cplx<T> apply_pin(const PassArgs<T>& a, cplx<T> v, co
 
 __device__ __forceinline__ double dens64(double g, double x, double y) { return g * (x * x + y * y); }
 
-template <typename T>
-__device__ __forceinline__ cplx<T> mid_op_lean(const PassArgs<T>& a, const Coord& c, cplx<T> v) {
-    switch (a.mid_mode) {
-    case MID_EXPT:
-    case MID_EXPTC: {
-        const bool cj = (a.mid_mode == MID_EXPTC);
-        if constexpr (sizeof(T) == 4) {
-            if (a.axis == 0 && a.kx_hl) {
-                // complex64 on the x axis: two compensated products with K_x and
-                // the per-pencil K_y K_z (hi + lo float pairs of FP64 values)
-                const unsigned yz = c.iy * (unsigned)a.ph.nz + c.iz;
-                cplx<float> h1 = a.kx_hl[2 * c.ix], l1 = a.kx_hl[2 * c.ix + 1];
-                cplx<float> h2 = a.kyz_hl[2 * yz], l2 = a.kyz_hl[2 * yz + 1];
-                if (cj) { h1.y = -h1.y; l1.y = -l1.y; h2.y = -h2.y; l2.y = -l2.y; }
-                v = cmul_comp(v, h1, l1);
-                return cmul_comp(v, h2, l2);
-            }
-        }
+// k-space factors of the lean passes, one per compile-time kind; MK_EXPT_HL is
+// the complex64 x-axis form of MID_EXPT / MID_EXPTC with the compensated
+// tables, MK_EXPT_64 the FP64 product
+enum MidKind { MK_EXPT_64 = 0, MK_EXPT_HL = 1, MK_K2 = 2, MK_VPP = 3, MK_KA = 4 };
+
+template <int MK, bool CJ, typename T>
+__device__ __forceinline__ cplx<T> mid_op_lean_k(const PassArgs<T>& a, const Coord& c, cplx<T> v) {
+    if constexpr (MK == MK_EXPT_HL && sizeof(T) == 4) {
+        // complex64 on the x axis: two compensated products with K_x and
+        // the per-pencil K_y K_z (hi + lo float pairs of FP64 values)
+        const unsigned yz = c.iy * (unsigned)a.ph.nz + c.iz;
+        cplx<float> h1 = a.kx_hl[2 * c.ix], l1 = a.kx_hl[2 * c.ix + 1];
+        cplx<float> h2 = a.kyz_hl[2 * yz], l2 = a.kyz_hl[2 * yz + 1];
+        if (CJ) { h1.y = -h1.y; l1.y = -l1.y; h2.y = -h2.y; l2.y = -l2.y; }
+        v = cmul_comp(v, h1, l1);
+        return cmul_comp(v, h2, l2);
+    } else if constexpr (MK == MK_EXPT_64 || MK == MK_EXPT_HL) {
         cplx<double> w = mk((double)v.x, (double)v.y);
         cplx<double> kx = a.Kx[c.ix], ky = a.Ky[c.iy], kz = a.Kz[c.iz];
-        if (cj) { kx.y = -kx.y; ky.y = -ky.y; kz.y = -kz.y; }
+        if (CJ) { kx.y = -kx.y; ky.y = -ky.y; kz.y = -kz.y; }
         w = cmul(w, kx);
         w = cmul(w, ky);
         w = cmul(w, kz);
         return mk((T)w.x, (T)w.y);
-    }
-    case MID_K2: {
+    } else if constexpr (MK == MK_K2) {
         const T k2 = a.kx[c.ix] * a.kx[c.ix] + a.ky[c.iy] * a.ky[c.iy] + a.kz[c.iz] * a.kz[c.iz];
         return cscale(v, a.kfac * k2);
-    }
-    case MID_VPP: {
+    } else if constexpr (MK == MK_VPP) {
         const double f = vpp_k(a.up64, a.kx64[c.ix], a.ky64[c.iy], a.kz64[c.iz]) /
                          ((double)a.ph.nx * (double)a.ph.ny * (double)a.ph.nz);
         return mk((T)((double)v.x * f), (T)((double)v.y * f));
-    }
-    case MID_KA: {
+    } else {
         const T k = (a.ka_axis == 0) ? a.kx[c.ix] : (a.ka_axis == 1) ? a.ky[c.iy] : a.kz[c.iz];
         return cscale(v, k / a.kfac);
     }
-    default:
-        return v;
+}
+
+// the kind and conjugation of a lean mid op; false when it has none
+template <typename T>
+__host__ __device__ __forceinline__ bool mid_kind(const PassArgs<T>& a, int& mk, bool& cj) {
+    cj = (a.mid_mode == MID_EXPTC);
+    switch (a.mid_mode) {
+    case MID_EXPT:
+    case MID_EXPTC:
+        mk = (sizeof(T) == 4 && a.axis == 0 && a.kx_hl) ? MK_EXPT_HL : MK_EXPT_64;
+        return true;
+    case MID_K2: mk = MK_K2; return true;
+    case MID_VPP: mk = MK_VPP; return true;
+    case MID_KA: mk = MK_KA; return true;
+    default: return false;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> mid_op_lean(const PassArgs<T>& a, const Coord& c, cplx<T> v) {
+    int mk;
+    bool cj;
+    if (!mid_kind(a, mk, cj)) return v;
+    switch (mk) {
+    case MK_EXPT_HL: return cj ? mid_op_lean_k<MK_EXPT_HL, true>(a, c, v) : mid_op_lean_k<MK_EXPT_HL, false>(a, c, v);
+    case MK_EXPT_64: return cj ? mid_op_lean_k<MK_EXPT_64, true>(a, c, v) : mid_op_lean_k<MK_EXPT_64, false>(a, c, v);
+    case MK_K2: return mid_op_lean_k<MK_K2, false>(a, c, v);
+    case MK_VPP: return mid_op_lean_k<MK_VPP, false>(a, c, v);
+    default: return mid_op_lean_k<MK_KA, false>(a, c, v);
     }
 }
 
@@ -666,17 +689,19 @@ struct Pencil {
     unsigned c0, c1;  // the two fixed coordinates
 };
 
-template <typename T>
+// AX >= 0: the axis known at compile time (no branch), -1: read from the arguments
+template <int AX = -1, typename T>
 __device__ __forceinline__ Pencil make_pencil(const PassArgs<T>& a, int N, unsigned p) {
     Pencil q;
     const unsigned nx = (unsigned)a.ph.nx, ny = (unsigned)a.ph.ny, nz = (unsigned)a.ph.nz;
-    if (a.axis == 2) {  // z pencils (b, ix, iy) in order: p = (b nx + ix) ny + iy
+    const int axis = AX >= 0 ? AX : a.axis;
+    if (axis == 2) {  // z pencils (b, ix, iy) in order: p = (b nx + ix) ny + iy
         const unsigned bx = p / ny;
         q.c1 = p - bx * ny;
         q.c0 = bx - (bx / nx) * nx;
         q.base = p * (unsigned)N;
         q.stride = 1;
-    } else if (a.axis == 1) {  // y pencils (b, ix, iz): p = (b nx + ix) nz + iz
+    } else if (axis == 1) {  // y pencils (b, ix, iz): p = (b nx + ix) nz + iz
         const unsigned bx = p / nz;
         q.c1 = p - bx * nz;
         q.c0 = bx - (bx / nx) * nx;
@@ -694,13 +719,17 @@ __device__ __forceinline__ Pencil make_pencil(const PassArgs<T>& a, int N, unsig
     return q;
 }
 
-template <typename T>
+// selects on the (uniform) axis, no branches: a branch per point would split
+// the unrolled per-point code into basic blocks the scheduler cannot interleave
+template <int AX = -1, typename T>
 __device__ __forceinline__ Coord elem(const PassArgs<T>& a, const Pencil& q, unsigned i) {
     Coord c;
     c.off = q.base + i * q.stride;
-    if (a.axis == 2) { c.ix = q.c0; c.iy = q.c1; c.iz = i; }
-    else if (a.axis == 1) { c.ix = q.c0; c.iy = i; c.iz = q.c1; }
-    else { c.ix = i; c.iy = q.c0; c.iz = q.c1; }
+    const int axis = AX >= 0 ? AX : a.axis;
+    const bool ax0 = (axis == 0), ax1 = (axis == 1);
+    c.ix = ax0 ? i : q.c0;
+    c.iy = ax0 ? q.c0 : (ax1 ? i : q.c1);
+    c.iz = (ax0 || ax1) ? q.c1 : i;
     return c;
 }
 
@@ -712,7 +741,7 @@ __device__ __forceinline__ void bmap(int b, int& w, int& j) {
 }
 
 // first stage from global memory (load op applied per element) into LDS
-template <typename T, int N, int W, bool CONTIG, bool FULL, int S, int R>
+template <typename T, int N, int W, bool CONTIG, bool FULL, int S, int R, int AX = -1>
 __device__ __forceinline__ void first_stage_global(const PassArgs<T>& a, unsigned p0, cplx<T>* lds, double* acc) {
     constexpr int M = N / R;
     constexpr int NB = W * M;
@@ -726,10 +755,10 @@ __device__ __forceinline__ void first_stage_global(const PassArgs<T>& a, unsigne
             const unsigned p = p0 + (unsigned)w;
             cplx<T> v[R];
             if (p < a.M) {
-                const Pencil pc = make_pencil(a, N, p);
+                const Pencil pc = make_pencil<AX>(a, N, p);
 #pragma unroll
                 for (int r = 0; r < R; r++) {
-                    const Coord c = elem(a, pc, (unsigned)(j + r * M));
+                    const Coord c = elem<AX>(a, pc, (unsigned)(j + r * M));
                     v[r] = a.src[c.off];
                 }
             } else {
@@ -803,7 +832,7 @@ __device__ __forceinline__ void last_stage_regs(const cplx<T>* lds, const cplx<T
 }
 
 // registers (w, j + r N/R) <- global memory with the load op (single-stage lengths)
-template <typename T, int N, int W, bool CONTIG, bool FULL, int R, int PER>
+template <typename T, int N, int W, bool CONTIG, bool FULL, int R, int PER, int AX = -1>
 __device__ __forceinline__ void load_regs(const PassArgs<T>& a, unsigned p0, cplx<T> (&v)[PER][R], double* acc) {
     constexpr int M = N / R;
     constexpr int NB = W * M;
@@ -814,11 +843,11 @@ __device__ __forceinline__ void load_regs(const PassArgs<T>& a, unsigned p0, cpl
         bmap<M, W, CONTIG>(b, w, j);
         const unsigned p = p0 + (unsigned)w;
         const bool ok = ((NB % kThreads) == 0 || b < NB) && p < a.M;
-        const Pencil pc = make_pencil(a, N, ok ? p : 0u);
+        const Pencil pc = make_pencil<AX>(a, N, ok ? p : 0u);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             if (ok) {
-                const Coord c = elem(a, pc, (unsigned)(j + r * M));
+                const Coord c = elem<AX>(a, pc, (unsigned)(j + r * M));
                 v[q][r] = a.src[c.off];
             } else {
                 v[q][r] = mk((T)0, (T)0);
@@ -827,10 +856,13 @@ __device__ __forceinline__ void load_regs(const PassArgs<T>& a, unsigned p0, cpl
     }
 }
 
-template <typename T, int N, int W, bool CONTIG, bool FULL, int R, int PER>
-__device__ __forceinline__ void mid_regs(const PassArgs<T>& a, unsigned p0, cplx<T> (&v)[PER][R], double* acc) {
+template <int MK, bool CJ, typename T, int N, int W, bool CONTIG, int R, int PER, int AX = -1>
+__device__ __forceinline__ void mid_regs_k(const PassArgs<T>& a, unsigned p0, cplx<T> (&v)[PER][R]) {
     constexpr int M = N / R;
     constexpr int NB = W * M;
+    // keep the factor-table loads here: hoisted above the first transform they
+    // would hold registers through it (a compiler-only barrier, no instruction)
+    asm volatile("" ::: "memory");
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         const int b = threadIdx.x + q * kThreads;
@@ -838,17 +870,17 @@ __device__ __forceinline__ void mid_regs(const PassArgs<T>& a, unsigned p0, cplx
         bmap<M, W, CONTIG>(b, w, j);
         const unsigned p = p0 + (unsigned)w;
         if (((NB % kThreads) == 0 || b < NB) && p < a.M) {
-            const Pencil pc = make_pencil(a, N, p);
+            const Pencil pc = make_pencil<AX>(a, N, p);
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                const Coord c = elem(a, pc, (unsigned)(j + r * M));
-                v[q][r] = mid_op_lean(a, c, v[q][r]);
+                const Coord c = elem<AX>(a, pc, (unsigned)(j + r * M));
+                v[q][r] = mid_op_lean_k<MK, CJ>(a, c, v[q][r]);
             }
         }
     }
 }
 
-template <typename T, int N, int W, bool CONTIG, int R, int PER>
+template <typename T, int N, int W, bool CONTIG, int R, int PER, int AX = -1>
 __device__ __forceinline__ void store_regs(const PassArgs<T>& a, unsigned p0, const cplx<T> (&v)[PER][R]) {
     constexpr int M = N / R;
     constexpr int NB = W * M;
@@ -859,7 +891,7 @@ __device__ __forceinline__ void store_regs(const PassArgs<T>& a, unsigned p0, co
         bmap<M, W, CONTIG>(b, w, j);
         const unsigned p = p0 + (unsigned)w;
         if (((NB % kThreads) == 0 || b < NB) && p < a.M) {
-            const Pencil pc = make_pencil(a, N, p);
+            const Pencil pc = make_pencil<AX>(a, N, p);
 #pragma unroll
             for (int r = 0; r < R; r++) a.dst[pc.base + (unsigned)(j + r * M) * pc.stride] = v[q][r];
         }
@@ -877,7 +909,8 @@ constexpr int lean_w() {
     return tile_w<T, N>() / WD > 0 ? tile_w<T, N>() / WD : 1;
 }
 
-template <typename T, int N, bool CONTIG, int S1, int S2, int WD>
+// MK: compile-time kind of the mid op (MidKind), -1 for none; CJ conjugates it
+template <typename T, int N, bool CONTIG, int S1, int S2, int WD, int MK, bool CJ, int AX>
 __device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, const cplx<T>* tw,
                                           const cplx<T>* twlo) {
     constexpr int W = lean_w<T, N, WD>();
@@ -888,16 +921,16 @@ __device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, co
     const unsigned p0 = blockIdx.x * (unsigned)W;
     cplx<T> v[PERL][RL];
     if constexpr (NSG == 1) {
-        load_regs<T, N, W, CONTIG, false, RL, PERL>(a, p0, v, nullptr);
+        load_regs<T, N, W, CONTIG, false, RL, PERL, AX>(a, p0, v, nullptr);
 #pragma unroll
         for (int q = 0; q < PERL; q++) dft<RL, S1>(v[q]);
     } else {
-        first_stage_global<T, N, W, CONTIG, false, S1, P.r[0]>(a, p0, lds, nullptr);
+        first_stage_global<T, N, W, CONTIG, false, S1, P.r[0], AX>(a, p0, lds, nullptr);
         __syncthreads();
         lds_stages<T, N, W, CONTIG, S1, false, 1, NSG - 1>(lds, tw, twlo);
         last_stage_regs<T, N, W, CONTIG, S1, RL, PERL>(lds, tw, twlo, v);
     }
-    if (a.mid_mode != MID_NONE) mid_regs<T, N, W, CONTIG, false, RL, PERL>(a, p0, v, nullptr);
+    if constexpr (MK >= 0) mid_regs_k<MK, CJ, T, N, W, CONTIG, RL, PERL, AX>(a, p0, v);
     if constexpr (S2 != 0) {
         if constexpr (NSG == 1) {
 #pragma unroll
@@ -913,11 +946,11 @@ __device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, co
             lds_stages<T, N, W, CONTIG, S2, true, 1, NSG - 1>(lds, tw, twlo);
             cplx<T> u[PERF][RF];
             last_stage_regs<T, N, W, CONTIG, S2, RF, PERF>(lds, tw, twlo, u);
-            store_regs<T, N, W, CONTIG, RF, PERF>(a, p0, u);
+            store_regs<T, N, W, CONTIG, RF, PERF, AX>(a, p0, u);
             return;
         }
     }
-    store_regs<T, N, W, CONTIG, RL, PERL>(a, p0, v);
+    store_regs<T, N, W, CONTIG, RL, PERL, AX>(a, p0, v);
 }
 
 // Physics passes.  Every per-element global input is gathered into LDS up
@@ -1105,8 +1138,9 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
     }
 }
 
-template <typename T, int N, bool CONTIG, bool FULL, int SHAPE, int WD = 1>
-__global__ __launch_bounds__(kThreads) void fft_pass_kernel(const PassArgs<T> a) {
+template <typename T, int N, bool CONTIG, bool FULL, int SHAPE, int WD = 1, int MK = -1, bool CJ = false, int AX = -1>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
+void fft_pass_kernel(const PassArgs<T> a) {
     __shared__ cplx<T> tw[N];
     __shared__ cplx<T> twlo[sizeof(T) == 4 ? N : 1];
     for (int i = threadIdx.x; i < N; i += kThreads) {
@@ -1120,7 +1154,7 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(const PassArgs<T> a)
     if constexpr (!FULL) {
         constexpr int W = lean_w<T, N, WD>();
         __shared__ cplx<T> lds[lds_elems<N, W, CONTIG>()];
-        lean_body<T, N, CONTIG, S1, S2, WD>(a, lds, tw, twlo);
+        lean_body<T, N, CONTIG, S1, S2, WD, MK, CJ, (CONTIG ? 2 : AX)>(a, lds, tw, twlo);
     } else {
         __shared__ FullLds<T, N, CONTIG> L;
         __shared__ double red[kThreads / 64];

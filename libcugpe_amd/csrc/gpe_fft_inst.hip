@@ -25,6 +25,25 @@ unsigned launch_fft_pass_n_impl(const PassArgs<T>& a, hipStream_t s) {
     const int shape = pass_shape(a.dir1, a.dir2);
 #define GPE_LAUNCH(CONTIG, FULL, SHAPE, WD) \
     hipLaunchKernelGGL((fft_pass_kernel<T, NN, CONTIG, FULL, SHAPE, WD>), dim3(blocks), dim3(kThreads), 0, s, a)
+// lean strided passes: the axis (0 x, 1 y) is a template argument
+#define GPE_LAUNCH_S(SHAPE, WD, MK, CJ)                                                                        \
+    do {                                                                                                       \
+        if (a.axis == 0)                                                                                       \
+            hipLaunchKernelGGL((fft_pass_kernel<T, NN, false, false, SHAPE, WD, MK, CJ, 0>), dim3(blocks),     \
+                               dim3(kThreads), 0, s, a);                                                       \
+        else                                                                                                   \
+            hipLaunchKernelGGL((fft_pass_kernel<T, NN, false, false, SHAPE, WD, MK, CJ, 1>), dim3(blocks),     \
+                               dim3(kThreads), 0, s, a);                                                       \
+    } while (0)
+// the turn-around with a mid op is always the x pass
+#define GPE_LAUNCH_MID(WD, MK, CJ)                                                                            \
+    hipLaunchKernelGGL((fft_pass_kernel<T, NN, false, false, SHAPE_FI, WD, MK, CJ, 0>), dim3(blocks),          \
+                       dim3(kThreads), 0, s, a)
+    int mk = -1;
+    bool cj = false;
+    const bool has_mid = lean && mid_kind(a, mk, cj);
+    // lean mid ops live in the turn-around of the x axis
+    if (lean && has_mid && (shape != SHAPE_FI || a.axis != 0)) return 0;
     if (a.axis == 2) {
         if (lean && shape == SHAPE_F) GPE_LAUNCH(true, false, SHAPE_F, 1);
         else if (lean && shape == SHAPE_I) GPE_LAUNCH(true, false, SHAPE_I, 1);
@@ -32,18 +51,43 @@ unsigned launch_fft_pass_n_impl(const PassArgs<T>& a, hipStream_t s) {
         else if (shape == SHAPE_I) GPE_LAUNCH(true, true, SHAPE_I, 1);
         else if (shape == SHAPE_IF) GPE_LAUNCH(true, true, SHAPE_IF, 1);
         else return 0;
-    } else if (half) {
-        if (shape == SHAPE_F) GPE_LAUNCH(false, false, SHAPE_F, 2);
-        else if (shape == SHAPE_I) GPE_LAUNCH(false, false, SHAPE_I, 2);
-        else if (shape == SHAPE_FI) GPE_LAUNCH(false, false, SHAPE_FI, 2);
-        else return 0;
+    } else if (lean && shape == SHAPE_FI) {
+        if (!has_mid) {
+            if (half) GPE_LAUNCH_S(SHAPE_FI, 2, -1, false);
+            else GPE_LAUNCH_S(SHAPE_FI, 1, -1, false);
+        } else if (half) {
+            switch (mk) {
+            case MK_EXPT_HL: if (cj) GPE_LAUNCH_MID(2, MK_EXPT_HL, true); else GPE_LAUNCH_MID(2, MK_EXPT_HL, false); break;
+            case MK_EXPT_64: if (cj) GPE_LAUNCH_MID(2, MK_EXPT_64, true); else GPE_LAUNCH_MID(2, MK_EXPT_64, false); break;
+            case MK_K2: GPE_LAUNCH_MID(2, MK_K2, false); break;
+            case MK_VPP: GPE_LAUNCH_MID(2, MK_VPP, false); break;
+            default: GPE_LAUNCH_MID(2, MK_KA, false); break;
+            }
+        } else {
+            switch (mk) {
+            case MK_EXPT_HL: if (cj) GPE_LAUNCH_MID(1, MK_EXPT_HL, true); else GPE_LAUNCH_MID(1, MK_EXPT_HL, false); break;
+            case MK_EXPT_64: if (cj) GPE_LAUNCH_MID(1, MK_EXPT_64, true); else GPE_LAUNCH_MID(1, MK_EXPT_64, false); break;
+            case MK_K2: GPE_LAUNCH_MID(1, MK_K2, false); break;
+            case MK_VPP: GPE_LAUNCH_MID(1, MK_VPP, false); break;
+            default: GPE_LAUNCH_MID(1, MK_KA, false); break;
+            }
+        }
+    } else if (lean) {
+        if (shape == SHAPE_F) {
+            if (half) GPE_LAUNCH_S(SHAPE_F, 2, -1, false);
+            else GPE_LAUNCH_S(SHAPE_F, 1, -1, false);
+        } else if (shape == SHAPE_I) {
+            if (half) GPE_LAUNCH_S(SHAPE_I, 2, -1, false);
+            else GPE_LAUNCH_S(SHAPE_I, 1, -1, false);
+        } else {
+            return 0;
+        }
     } else {
-        if (lean && shape == SHAPE_F) GPE_LAUNCH(false, false, SHAPE_F, 1);
-        else if (lean && shape == SHAPE_I) GPE_LAUNCH(false, false, SHAPE_I, 1);
-        else if (lean && shape == SHAPE_FI) GPE_LAUNCH(false, false, SHAPE_FI, 1);
-        else if (!lean && shape == SHAPE_F) GPE_LAUNCH(false, true, SHAPE_F, 1);
+        if (shape == SHAPE_F) GPE_LAUNCH(false, true, SHAPE_F, 1);
         else return 0;
     }
+#undef GPE_LAUNCH_S
+#undef GPE_LAUNCH_MID
 #undef GPE_LAUNCH
     return blocks;
 }
