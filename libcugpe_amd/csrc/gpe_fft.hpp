@@ -140,6 +140,31 @@ __device__ __forceinline__ cplx<float> cmul_comp(cplx<float> a, cplx<float> hi, 
               fmaf(a.x, hi.y, fmaf(a.y, hi.x, a.x * lo.y + a.y * lo.x)));
 }
 
+// A twiddle entry in LDS: complex64 keeps the float hi and lo parts of
+// exp(-2 pi i m / N) together (one 16-byte read), complex128 the value.
+template <typename T>
+struct TwE;
+template <>
+struct TwE<float> {
+    float4 t;  // hi.x, hi.y, lo.x, lo.y
+};
+template <>
+struct TwE<double> {
+    cplx<double> t;
+};
+// a * w for sign S < 0, a * conj(w) for S > 0
+template <int S>
+__device__ __forceinline__ cplx<float> tw_mul(cplx<float> a, const TwE<float>& e) {
+    const float4 q = e.t;
+    return cmul_comp(a, mk(q.x, S > 0 ? -q.y : q.y), mk(q.z, S > 0 ? -q.w : q.w));
+}
+template <int S>
+__device__ __forceinline__ cplx<double> tw_mul(cplx<double> a, const TwE<double>& e) {
+    cplx<double> t = e.t;
+    if (S > 0) t.y = -t.y;
+    return cmul(a, t);
+}
+
 template <int R, int S, typename T>
 __device__ __forceinline__ void dft(cplx<T>* v);
 
@@ -227,7 +252,7 @@ __device__ __forceinline__ void dft(cplx<T>* v) {
 // (w, j + r N/R), applies w_N^(r (j mod NS) N/(NS R)), transforms, and writes
 // (w, (j/NS) NS R + j mod NS + r NS).
 template <typename T, int N, int W, bool CONTIG, int S, int R, int NS>
-__device__ __forceinline__ void fft_stage(cplx<T>* lds, const cplx<T>* tw, const cplx<T>* twlo) {
+__device__ __forceinline__ void fft_stage(cplx<T>* lds, const TwE<T>* tw) {
     constexpr int NB = W * N / R;
     constexpr int PER = (NB + kThreads - 1) / kThreads;
     constexpr int M = N / R;
@@ -256,17 +281,7 @@ __device__ __forceinline__ void fft_stage(cplx<T>* lds, const cplx<T>* tw, const
             const int k = j % NS;
             if constexpr (NS > 1) {
 #pragma unroll
-                for (int r = 1; r < R; r++) {
-                    cplx<T> t = tw[k * r * TSTEP];
-                    if (S > 0) t.y = -t.y;
-                    if constexpr (sizeof(T) == 4) {
-                        cplx<T> tl = twlo[k * r * TSTEP];
-                        if (S > 0) tl.y = -tl.y;
-                        v[q][r] = cmul_comp(v[q][r], t, tl);
-                    } else {
-                        v[q][r] = cmul(v[q][r], t);
-                    }
-                }
+                for (int r = 1; r < R; r++) v[q][r] = tw_mul<S>(v[q][r], tw[k * r * TSTEP]);
             }
             dft<R, S>(v[q]);
             const int idxD = (j / NS) * NS * R + k;
@@ -278,11 +293,11 @@ __device__ __forceinline__ void fft_stage(cplx<T>* lds, const cplx<T>* tw, const
 }
 
 template <typename T, int N, int W, bool CONTIG, int S, int STAGE>
-__device__ __forceinline__ void fft_stages(cplx<T>* lds, const cplx<T>* tw, const cplx<T>* twlo) {
+__device__ __forceinline__ void fft_stages(cplx<T>* lds, const TwE<T>* tw) {
     constexpr FftPlan P = make_plan(N);
     if constexpr (STAGE < P.ns) {
-        fft_stage<T, N, W, CONTIG, S, P.r[STAGE], plan_span(P, STAGE)>(lds, tw, twlo);
-        fft_stages<T, N, W, CONTIG, S, STAGE + 1>(lds, tw, twlo);
+        fft_stage<T, N, W, CONTIG, S, P.r[STAGE], plan_span(P, STAGE)>(lds, tw);
+        fft_stages<T, N, W, CONTIG, S, STAGE + 1>(lds, tw);
     }
 }
 
@@ -792,17 +807,17 @@ __device__ __forceinline__ void first_stage_regs(cplx<T> (&v)[PER][R], cplx<T>* 
 
 // stages [STAGE, LAST) of plan (N, REV) in LDS
 template <typename T, int N, int W, bool CONTIG, int S, bool REV, int STAGE, int LAST>
-__device__ __forceinline__ void lds_stages(cplx<T>* lds, const cplx<T>* tw, const cplx<T>* twlo) {
+__device__ __forceinline__ void lds_stages(cplx<T>* lds, const TwE<T>* tw) {
     if constexpr (STAGE < LAST) {
         constexpr FftPlan P = plan_of(N, REV);
-        fft_stage<T, N, W, CONTIG, S, P.r[STAGE], plan_span(P, STAGE)>(lds, tw, twlo);
-        lds_stages<T, N, W, CONTIG, S, REV, STAGE + 1, LAST>(lds, tw, twlo);
+        fft_stage<T, N, W, CONTIG, S, P.r[STAGE], plan_span(P, STAGE)>(lds, tw);
+        lds_stages<T, N, W, CONTIG, S, REV, STAGE + 1, LAST>(lds, tw);
     }
 }
 
 // last stage (span N/R) from LDS into registers: v[q][r] is element (w, j + r N/R)
 template <typename T, int N, int W, bool CONTIG, int S, int R, int PER>
-__device__ __forceinline__ void last_stage_regs(const cplx<T>* lds, const cplx<T>* tw, const cplx<T>* twlo,
+__device__ __forceinline__ void last_stage_regs(const cplx<T>* lds, const TwE<T>* tw,
                                                 cplx<T> (&v)[PER][R]) {
     constexpr int M = N / R;
     constexpr int NB = W * M;
@@ -815,17 +830,7 @@ __device__ __forceinline__ void last_stage_regs(const cplx<T>* lds, const cplx<T
 #pragma unroll
             for (int r = 0; r < R; r++) v[q][r] = lds[lidx<N, W, CONTIG>(w, j + r * M)];
 #pragma unroll
-            for (int r = 1; r < R; r++) {  // twiddle w_N^(j r)
-                cplx<T> t = tw[j * r];
-                if (S > 0) t.y = -t.y;
-                if constexpr (sizeof(T) == 4) {
-                    cplx<T> tl = twlo[j * r];
-                    if (S > 0) tl.y = -tl.y;
-                    v[q][r] = cmul_comp(v[q][r], t, tl);
-                } else {
-                    v[q][r] = cmul(v[q][r], t);
-                }
-            }
+            for (int r = 1; r < R; r++) v[q][r] = tw_mul<S>(v[q][r], tw[j * r]);  // twiddle w_N^(j r)
             dft<R, S>(v[q]);
         }
     }
@@ -911,8 +916,7 @@ constexpr int lean_w() {
 
 // MK: compile-time kind of the mid op (MidKind), -1 for none; CJ conjugates it
 template <typename T, int N, bool CONTIG, int S1, int S2, int WD, int MK, bool CJ, int AX>
-__device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, const cplx<T>* tw,
-                                          const cplx<T>* twlo) {
+__device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, const TwE<T>* tw) {
     constexpr int W = lean_w<T, N, WD>();
     constexpr FftPlan P = plan_of(N, false);
     constexpr int NSG = P.ns;
@@ -927,8 +931,8 @@ __device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, co
     } else {
         first_stage_global<T, N, W, CONTIG, false, S1, P.r[0], AX>(a, p0, lds, nullptr);
         __syncthreads();
-        lds_stages<T, N, W, CONTIG, S1, false, 1, NSG - 1>(lds, tw, twlo);
-        last_stage_regs<T, N, W, CONTIG, S1, RL, PERL>(lds, tw, twlo, v);
+        lds_stages<T, N, W, CONTIG, S1, false, 1, NSG - 1>(lds, tw);
+        last_stage_regs<T, N, W, CONTIG, S1, RL, PERL>(lds, tw, v);
     }
     if constexpr (MK >= 0) mid_regs_k<MK, CJ, T, N, W, CONTIG, RL, PERL, AX>(a, p0, v);
     if constexpr (S2 != 0) {
@@ -943,9 +947,9 @@ __device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, co
             __syncthreads();  // the first transform's last-stage reads are done
             first_stage_regs<T, N, W, CONTIG, S2, RL, PERL>(v, lds);
             __syncthreads();
-            lds_stages<T, N, W, CONTIG, S2, true, 1, NSG - 1>(lds, tw, twlo);
+            lds_stages<T, N, W, CONTIG, S2, true, 1, NSG - 1>(lds, tw);
             cplx<T> u[PERF][RF];
-            last_stage_regs<T, N, W, CONTIG, S2, RF, PERF>(lds, tw, twlo, u);
+            last_stage_regs<T, N, W, CONTIG, S2, RF, PERF>(lds, tw, u);
             store_regs<T, N, W, CONTIG, RF, PERF, AX>(a, p0, u);
             return;
         }
@@ -1019,8 +1023,7 @@ __device__ __forceinline__ ElemIn<T> elem_in(const PassArgs<T>& a, const FullLds
 }
 
 template <typename T, int N, bool CONTIG, int S1, int S2>
-__device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CONTIG>& L, const cplx<T>* tw,
-                                          const cplx<T>* twlo, double* acc) {
+__device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CONTIG>& L, const TwE<T>* tw, double* acc) {
     constexpr int W = FullLds<T, N, CONTIG>::W;
     constexpr int TILE = W * N;
     constexpr int PERE = (TILE + kThreads - 1) / kThreads;
@@ -1102,12 +1105,12 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
 
     cplx<T> v[PERL][RL];
     if (a.mid_mode == MID_NONE) {
-        lds_stages<T, N, W, CONTIG, S1, false, 0, NSG - 1>(L.main, tw, twlo);
-        last_stage_regs<T, N, W, CONTIG, S1, RL, PERL>(L.main, tw, twlo, v);
+        lds_stages<T, N, W, CONTIG, S1, false, 0, NSG - 1>(L.main, tw);
+        last_stage_regs<T, N, W, CONTIG, S1, RL, PERL>(L.main, tw, v);
         if (a.store_mode == STORE_PLAIN) store_regs<T, N, W, CONTIG, RL, PERL>(a, p0, v);
         return;
     }
-    lds_stages<T, N, W, CONTIG, S1, false, 0, NSG>(L.main, tw, twlo);
+    lds_stages<T, N, W, CONTIG, S1, false, 0, NSG>(L.main, tw);
     for (int q = 0; q < PERE; q++) {  // mid op, natural order
         const int e = threadIdx.x + q * kThreads;
         if ((TILE % kThreads) == 0 || e < TILE) {
@@ -1122,8 +1125,8 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
     }
     __syncthreads();
     if constexpr (S2 != 0) {
-        lds_stages<T, N, W, CONTIG, S2, false, 0, NSG - 1>(L.main, tw, twlo);
-        last_stage_regs<T, N, W, CONTIG, S2, RL, PERL>(L.main, tw, twlo, v);
+        lds_stages<T, N, W, CONTIG, S2, false, 0, NSG - 1>(L.main, tw);
+        last_stage_regs<T, N, W, CONTIG, S2, RL, PERL>(L.main, tw, v);
         if (a.store_mode == STORE_PLAIN) store_regs<T, N, W, CONTIG, RL, PERL>(a, p0, v);
     } else if (a.store_mode == STORE_PLAIN) {
         for (int q = 0; q < PERE; q++) {
@@ -1141,11 +1144,14 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
 template <typename T, int N, bool CONTIG, bool FULL, int SHAPE, int WD = 1, int MK = -1, bool CJ = false, int AX = -1>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
 void fft_pass_kernel(const PassArgs<T> a) {
-    __shared__ cplx<T> tw[N];
-    __shared__ cplx<T> twlo[sizeof(T) == 4 ? N : 1];
+    __shared__ TwE<T> tw[N];
     for (int i = threadIdx.x; i < N; i += kThreads) {
-        tw[i] = a.tw[i];
-        if constexpr (sizeof(T) == 4) twlo[i] = a.twlo[i];
+        if constexpr (sizeof(T) == 4) {
+            const cplx<float> h = a.tw[i], lo = a.twlo[i];
+            tw[i].t = make_float4(h.x, h.y, lo.x, lo.y);
+        } else {
+            tw[i].t = a.tw[i];
+        }
     }
     // (the first barrier of every body orders these table writes before any use;
     // a single-stage length reads no table)
@@ -1154,12 +1160,12 @@ void fft_pass_kernel(const PassArgs<T> a) {
     if constexpr (!FULL) {
         constexpr int W = lean_w<T, N, WD>();
         __shared__ cplx<T> lds[lds_elems<N, W, CONTIG>()];
-        lean_body<T, N, CONTIG, S1, S2, WD, MK, CJ, (CONTIG ? 2 : AX)>(a, lds, tw, twlo);
+        lean_body<T, N, CONTIG, S1, S2, WD, MK, CJ, (CONTIG ? 2 : AX)>(a, lds, tw);
     } else {
         __shared__ FullLds<T, N, CONTIG> L;
         __shared__ double red[kThreads / 64];
         double acc[3] = {0.0, 0.0, 0.0};
-        full_body<T, N, CONTIG, S1, S2>(a, L, tw, twlo, acc);
+        full_body<T, N, CONTIG, S1, S2>(a, L, tw, acc);
         // FP64 partial sums of this workgroup (indices fixed at compile time)
         if (a.nred > 0) {
             const double s0 = block_sum(acc[0], red);

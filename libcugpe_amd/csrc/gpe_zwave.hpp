@@ -18,6 +18,8 @@
 // proceed independently and overlap their loads with other waves' arithmetic.
 #pragma once
 
+#include <type_traits>
+
 #include "gpe_fft.hpp"
 
 namespace gpe {
@@ -41,7 +43,9 @@ struct ZW {
     static constexpr int B = N / A;
     static constexpr int PW = 64 / A;             // pencils per wave
     static constexpr int ROW = A + 1;             // padded exchange row
-    static constexpr int GS = B * ROW + 2;        // exchange elements per pencil
+    // exchange elements per pencil, = 16 mod 32 so that consecutive pencils of
+    // a wave start half a bank cycle apart (8-byte elements, 64 banks)
+    static constexpr int GS = B * ROW + ((16 - (B * ROW) % 32) + 32) % 32;
     static constexpr int WAVE = PW * GS;          // exchange elements per wave
     static constexpr int WAVES = kThreads / 64;
     static constexpr int PENCILS = PW * WAVES;    // pencils per workgroup
@@ -50,32 +54,33 @@ struct ZW {
 // load mode of the energy pass: plain load plus the potential-energy sums
 constexpr int LOAD_REDPOT = 3;
 
+// f(integral_constant<int, I>) for I = B .. E-1, each call a separate instance so
+// that indices derived from I are compile-time constants
+template <int I, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (I < E) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, E>(f);
+    }
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// twiddle w_N^(l kb) of lane l at row kb is entry kb A + l, so the lanes of a
+// pencil read consecutive entries
+
 // one transform of sign S on the register layout n = l + A b
 template <typename T, int N, int S, int B>
-__device__ __forceinline__ void zw_fft(cplx<T> (&v)[B], cplx<T>* X, const cplx<T>* tw, const cplx<T>* twlo,
-                                       int l) {
+__device__ __forceinline__ void zw_fft(cplx<T> (&v)[B], cplx<T>* X, const TwE<T>* tw, int l) {
     constexpr int A = ZW<T, N>::A, ROW = ZW<T, N>::ROW;
     static_assert(A * B == N, "lane split");
     dft<B, S>(v);
 #pragma unroll
-    for (int kb = 1; kb < B; kb++) {  // w_N^(l kb)
-        const int m = l * kb;
-        cplx<T> t = tw[m];
-        if (S > 0) t.y = -t.y;
-        if constexpr (sizeof(T) == 4) {
-            cplx<T> tl = twlo[m];
-            if (S > 0) tl.y = -tl.y;
-            v[kb] = cmul_comp(v[kb], t, tl);
-        } else {
-            v[kb] = cmul(v[kb], t);
-        }
-    }
+    for (int kb = 1; kb < B; kb++) v[kb] = tw_mul<S>(v[kb], tw[kb * A + l]);  // w_N^(l kb)
 #pragma unroll
     for (int kb = 0; kb < B; kb++) X[kb * ROW + l] = v[kb];
     wave_lds_sync();
@@ -93,8 +98,7 @@ __device__ __forceinline__ void zw_fft(cplx<T> (&v)[B], cplx<T>* X, const cplx<T
 
 template <typename T, int N>
 struct ZwLds {
-    cplx<T> tw[N];
-    cplx<T> twlo[sizeof(T) == 4 ? N : 1];
+    TwE<T> tw[N];
     T vz[N];
     cplx<T> x[ZW<T, N>::WAVES * ZW<T, N>::WAVE];
     double red[kThreads / 64];
@@ -181,8 +185,13 @@ void zwave_kernel(const PassArgs<T> a) {
     const int vk = a.ph.vext;
     const bool sep = (vk == VEXT_HARMONIC || vk == VEXT_COSINE);
     for (int i = threadIdx.x; i < N; i += kThreads) {
-        L.tw[i] = a.tw[i];
-        if constexpr (sizeof(T) == 4) L.twlo[i] = a.twlo[i];
+        const int m = (i / A) * (i % A);  // row kb = i / A, lane l = i % A
+        if constexpr (sizeof(T) == 4) {
+            const cplx<float> h = a.tw[m], lo = a.twlo[m];
+            L.tw[i].t = make_float4(h.x, h.y, lo.x, lo.y);
+        } else {
+            L.tw[i].t = a.tw[m];
+        }
         if (sep) L.vz[i] = a.ph.vtab[a.ph.nx + a.ph.ny + i];
     }
     __syncthreads();
@@ -202,10 +211,10 @@ void zwave_kernel(const PassArgs<T> a) {
     const T vscale = (vk == VEXT_COSINE) ? a.ph.up[0] : (T)1;
     const T vxy = sep ? a.ph.vtab[c.ix] + a.ph.vtab[a.ph.nx + c.iy] : (T)0;
     double acc[3] = {0.0, 0.0, 0.0};
-    auto off_of = [&](int b) { return base + (unsigned)(A * b); };
+    auto off_of = [&](int b) __attribute__((always_inline)) { return base + (unsigned)(A * b); };
 
     // the external potential and the coordinates of register b
-    auto elem_at = [&](int b) {
+    auto elem_at = [&](int b) __attribute__((always_inline)) {
         ElemIn<T> e;
         e.c = c;
         e.c.iz = (unsigned)(l + A * b);
@@ -234,7 +243,9 @@ void zwave_kernel(const PassArgs<T> a) {
 #ifndef ZW_PREFETCH
 #define ZW_PREFETCH 1
 #endif
-    constexpr int CH = (B >= ZW_CH) ? ZW_CH : B;
+    // complex128 holds twice the registers per point: chunks of two
+    constexpr int CHT = sizeof(T) == 4 ? ZW_CH : 2;
+    constexpr int CH = (B >= CHT) ? CHT : B;
     constexpr int NCH = B / CH;
     constexpr bool PRE = (LM != LOAD_PLAIN);   // physics before the first transform
     constexpr bool POST = (MM != MID_NONE);    // physics after it
@@ -247,7 +258,7 @@ void zwave_kernel(const PassArgs<T> a) {
                       (LM == LOAD_PSI0 && !(a.flags & F_IMPRINT));
     cplx<T> ax[2][CH];
     T rx[2][CH];
-    auto fetch = [&](int q, int slot) {
+    auto fetch = [&](int q, int slot) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < CH; j++) {
             const unsigned off = base + (unsigned)(A * (q * CH + j));
@@ -274,7 +285,7 @@ void zwave_kernel(const PassArgs<T> a) {
         }
     };
     // one chunk of physics on v[q CH .. q CH + CH)
-    auto physics = [&](int q, int slot) {
+    auto physics = [&](int q, int slot) __attribute__((always_inline)) {
         if (fast) {
             T vx[CH], gv[CH], lv[CH];
             cplx<T> u[CH];
@@ -355,24 +366,26 @@ void zwave_kernel(const PassArgs<T> a) {
 
     constexpr int S1 = (SHAPE == SHAPE_F || SHAPE == SHAPE_FI) ? -1 : 1;
     constexpr int S2 = (SHAPE == SHAPE_FI) ? 1 : (SHAPE == SHAPE_IF) ? -1 : 0;
-    if constexpr (PRE || (POST && ZW_PREFETCH)) fetch(0, 0);
+    // complex128: the first chunk's side loads wait for the transform (registers)
+    constexpr bool EARLY = ZW_PREFETCH && sizeof(T) == 4;
+    if constexpr (PRE || (POST && EARLY)) fetch(0, 0);
     if constexpr (PRE) {
-#pragma unroll
-        for (int q = 0; q < NCH; q++) {
-            if (q + 1 < NCH) fetch(q + 1, (q + 1) & 1);
+        static_for<0, NCH>([&](auto qc) __attribute__((always_inline)) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (q + 1 < NCH) fetch(q + 1, (q + 1) & 1);
             physics(q, q & 1);
-        }
+        });
     }
-    zw_fft<T, N, S1, B>(v, X, L.tw, L.twlo, l);
-    if constexpr (POST && !ZW_PREFETCH) fetch(0, 0);
+    zw_fft<T, N, S1, B>(v, X, L.tw, l);
+    if constexpr (POST && !EARLY) fetch(0, 0);
     if constexpr (POST) {
-#pragma unroll
-        for (int q = 0; q < NCH; q++) {
-            if (q + 1 < NCH) fetch(q + 1, (q + 1) & 1);
+        static_for<0, NCH>([&](auto qc) __attribute__((always_inline)) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (q + 1 < NCH) fetch(q + 1, (q + 1) & 1);
             physics(q, q & 1);
-        }
+        });
     }
-    if constexpr (S2 != 0) zw_fft<T, N, S2, B>(v, X, L.tw, L.twlo, l);
+    if constexpr (S2 != 0) zw_fft<T, N, S2, B>(v, X, L.tw, l);
 
     if (a.store_mode == STORE_PLAIN && ok) {
 #pragma unroll
