@@ -322,6 +322,137 @@ def adjoint_bench(E, T, oracle_steps, fd_bumps=(0, 101, 307)):
     }
 
 
+def splitmix64_uniform(seed, n):
+    """n uniforms in [0, 1) from SplitMix64 (the same stream on any host)."""
+    k = np.arange(1, n + 1, dtype=np.uint64)
+    z = np.uint64(seed) + k * np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def config1_bench(E):
+    """Imaginary-time ground state of a trapped contact BEC, 32^3 complex128:
+    box L = 20 (oscillator units, dx = 0.625), V = r^2/2, N = 1e4, g = 1.25
+    (read as 1.0 N / L^3), Gaussian start of width 2 times (1 + 0.05 U) with U
+    seeded uniform, dt = 1e-3, 2000 steps normalised every step.  Lattice
+    units: dt_l = dt/dx^2, omega_l = omega dx^2, g_l = g/dx (a_l = g_l/4pi)."""
+    from time import perf_counter
+
+    from oracle import pyoracle
+
+    n1, L, N, g, dt, steps = 32, 20.0, 1.0e4, 1.25, 1e-3, 2000
+    dx = L / n1
+    shape = (n1,) * 3
+    up = [dx * dx] * 3 + [g / dx / (4 * np.pi)]
+    c = (np.arange(n1) - n1 // 2) * dx
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    psi0 = np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / (2 * 2.0 ** 2)) * (1 + 0.05 * splitmix64_uniform(1, n1 ** 3).reshape(shape))
+    psi0 = psi0.astype(np.complex128)
+    g128 = E.Engine(shape, 0.0, 1.0, dt / dx ** 2, N, E.CONTACT, E.HARMONIC, 0, E.C128)
+    o = pyoracle.OracleEngine(shape, 0.0, 1.0, dt / dx ** 2, N, -1, 0, 0)
+    for x in (g128, o):
+        x.set_user_params(up)
+        x.set_psi(psi0)
+        x.normalize_psi()
+    g128.synchronize()
+    t0 = perf_counter()
+    mu_g = g128.evolve_qf(steps, want_mu=True)
+    t_g = perf_counter() - t0
+    t0 = perf_counter()
+    mu_o = o.evolve_qf(steps, want_mu=True)
+    t_o = perf_counter() - t0
+    eg, eo = g128.energy(), o.energy()
+    Eg, Eo = sum(eg[1:]), sum(eo[1:])
+    rg, _ = g128.get_density()
+    ro, _ = o.get_density()
+    g128.close()
+    o.close()
+    return {
+        "workload": "32^3 complex128 imaginary time, contact BEC N=1e4 g=1.25, V=r^2/2, L=20, dt=1e-3, 2000 steps",
+        "steps_per_s": steps / t_g,
+        "oracle_steps_per_s": steps / t_o,
+        "energy_per_particle_osc_units": Eg / N / dx ** 2,
+        "mu_osc_units": mu_g / dx ** 2,
+        "energy_rel_err_vs_fp64": abs(Eg - Eo) / abs(Eo),
+        "mu_rel_err_vs_fp64": abs(mu_g - mu_o) / abs(mu_o),
+        "density_max_rel_err_vs_fp64": float(np.max(np.abs(rg - ro)) / np.max(ro)),
+        "density_l2_rel_err_vs_fp64": float(np.linalg.norm(rg - ro) / np.linalg.norm(ro)),
+    }
+
+
+def config2_bench(E, steps, oracle_steps):
+    """Real-time unitary Fermi gas (dimers, xi = 0.37) with quantum friction
+    0.1 in an anisotropic trap omega = (1, 1, 0.5), 128^3, box L = 32 (dx =
+    0.25), dt = 5e-4, from a seeded Thomas-Fermi density with one vortex line
+    along z imprinted as exp(i theta).  complex128 and complex64 on the full
+    run; the FP64 oracle on the first oracle_steps steps for the windings
+    (bit-exact) and the energies."""
+    from time import perf_counter
+
+    from oracle import pyoracle
+
+    n1, L, N, dt = 128, 32.0, 1.0e4, 5e-4
+    dx = L / n1
+    shape = (n1,) * 3
+    om = (1.0, 1.0, 0.5)
+    up = [w * dx * dx for w in om]
+    c = (np.arange(n1) - n1 // 2) * dx
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    V = 0.5 * (om[0] ** 2 * X ** 2 + om[1] ** 2 * Y ** 2 + om[2] ** 2 * Z ** 2)
+    rho = np.maximum(1.0 - V / 25.0, 0.0) ** 1.5 * (1 + 0.01 * splitmix64_uniform(2, n1 ** 3).reshape(shape))
+    theta = np.arctan2(Y + 0.5 * dx, X + 0.5 * dx)  # the line between lattice points
+    psi0 = (np.sqrt(rho) * np.exp(1j * theta)).astype(np.complex128)
+    out = {"workload": "128^3 real-time UFG (dimers), QF 0.1, trap (1,1,0.5), L=32, dt=5e-4, %d steps, vortex line" % steps}
+    res = {}
+    for prec, name in ((E.C128, "c128"), (E.C64, "c64")):
+        g = E.Engine(shape, 1.0, 0.0, dt / dx ** 2, N, E.UFG, E.HARMONIC, 0, prec)
+        g.set_user_params(up)
+        g.set_quantum_friction_coeff(0.1)
+        g.set_psi(psi0)
+        g.normalize_psi()
+        _, a, b, cc = g.energy()
+        g.synchronize()
+        t0 = perf_counter()
+        g.evolve_qf(steps)
+        g.synchronize()
+        tt = perf_counter() - t0
+        _, a2, b2, c2 = g.energy()
+        pf, _ = g.get_psi()
+        _, cnt = g.detect_vortices(1e-2 * float(np.max(np.abs(pf) ** 2)))
+        res[name] = (a + b + cc, a2 + b2 + c2)
+        out[name + "_steps_per_s"] = steps / tt
+        out[name + "_energy_change"] = (a2 + b2 + c2) / (a + b + cc) - 1
+        out[name + "_vortex_plaquettes"] = int(cnt)
+        g.close()
+    out["energy_rel_err_c64_vs_c128_end"] = abs(res["c64"][1] - res["c128"][1]) / abs(res["c128"][1])
+    # the oracle on a bounded sample: windings bit-exact, energy
+    g = E.Engine(shape, 1.0, 0.0, dt / dx ** 2, N, E.UFG, E.HARMONIC, 0, E.C128)
+    o = pyoracle.OracleEngine(shape, 1.0, 0.0, dt / dx ** 2, N, 0, 0, 0)
+    for x in (g, o):
+        x.set_user_params(up)
+        x.set_quantum_friction_coeff(0.1)
+        x.set_psi(psi0)
+        x.normalize_psi()
+    g.evolve_qf(oracle_steps)
+    o.evolve_qf(oracle_steps)
+    po, _ = o.get_psi()
+    rmin = 1e-2 * float(np.max(np.abs(po) ** 2))
+    wg, cg = g.detect_vortices(rmin)
+    wo, co = pyoracle.plaquette_windings(po, rmin)
+    eg, eo = g.energy(), o.energy()
+    g.close()
+    o.close()
+    out["oracle_sample"] = {
+        "steps": oracle_steps,
+        "windings_bit_exact": bool(np.array_equal(wg, wo)),
+        "vortex_plaquettes": int(cg),
+        "energy_rel_err_c128_vs_fp64": abs(sum(eg[1:]) - sum(eo[1:])) / abs(sum(eo[1:])),
+    }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -334,6 +465,9 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--config2-steps", type=int, default=10000)
+    ap.add_argument("--config2-oracle-steps", type=int, default=40)
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-oracle-steps", type=int, default=200)
     a = ap.parse_args()
@@ -449,6 +583,13 @@ def main():
     if not a.no_ensemble:
         print("[bench] batched ensemble (%d steps)" % a.ensemble_steps, file=sys.stderr, flush=True)
         result["ensemble"] = ensemble_bench(E, a.ensemble_steps, 50, 8, 100)
+
+    if not a.no_configs:
+        print("[bench] config 1: 32^3 complex128 ground state", file=sys.stderr, flush=True)
+        result["config1"] = config1_bench(E)
+        print("[bench] config 2: 128^3 UFG with quantum friction (%d steps)" % a.config2_steps, file=sys.stderr,
+              flush=True)
+        result["config2"] = config2_bench(E, a.config2_steps, a.config2_oracle_steps)
 
     if not a.no_adjoint:
         print("[bench] adjoint gradient (T=%d)" % a.adjoint_steps, file=sys.stderr, flush=True)
