@@ -223,6 +223,12 @@ const char *gpe_build_info(void);
  * current state (restored afterwards) and keeps the fastest. */
 int gpe_engine_autotune(gpe_engine_t e, int steps, int *variant);
 int gpe_engine_set_variant(gpe_engine_t e, int variant);
+/* 1: an evolve call of 3 or more steps replays HIP graphs of the open pass,
+ * a middle step and the last step, captured once per step plan and recaptured
+ * after any parameter change; 0 (default, measured faster on one stream):
+ * launch every pass directly.  The results are the same bit for bit (no
+ * counterpart in the reference). */
+int gpe_engine_set_graph(gpe_engine_t e, int on);
 
 /* Adjoint of a real-time run (no counterpart in the reference).  forward:
  * nt steps of gpe_evolve_qf from the current psi, keeping every ckpt-th state

@@ -90,6 +90,14 @@ struct gpe_engine_s {
     bool adj_ready = false;
     double adj_fr = 0, adj_fi = 0;  // lambda(T) = (fr + i fi) psi_target
 
+    // captured step graphs (open, middle step, last step) for one step plan;
+    // gen counts parameter changes, a graph built at another gen is stale
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    int graph_pin = -1;
+    bool graph_mid_pin = false;
+    unsigned graph_gen = 0, gen = 1;
+    bool use_graph = false;  // measured: direct launches are faster on one stream
+
     // profiling
     int prof_class = -1;
     std::vector<EventPair> prof_ev;
@@ -327,6 +335,14 @@ int run_pass(gpe_engine_s* e, PassArgs<T> a, const PassSpec& s) {
     return (int)blocks;
 }
 
+void drop_graphs(gpe_engine_s* e) {
+    for (auto& g : e->graph) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    e->graph_gen = 0;
+}
+
 void finalize(gpe_engine_s* e, int nb, int nq, int mode, int slot, double div) {
     launch_finalize(e->partials, nb, nq, e->scal, mode, slot, e->npart, e->dt, div, e->stream);
 }
@@ -434,10 +450,37 @@ int evolve_t(gpe_engine_s* e, int nt, double* mu, int pin, bool mid_pin) {
     StepPlan sp;
     sp.pin = pin;
     sp.mid_pin = mid_pin;
-    issue_open<T>(e, sp);
-    for (int s = 0; s < nt; s++) {
-        issue_step<T>(e, sp, s == nt - 1);
-        e->it = e->it + 1;
+    // With graphs on, runs of 3+ steps replay captured graphs: the passes' arguments are the
+    // same every step (the time index among them is read by no built-in
+    // potential), so one capture of the open pass, a middle step and the last
+    // step serves every run until a parameter changes.  Not while profiling.
+    if (e->use_graph && nt >= 3 && e->prof_class < 0 && e->batch == 1) {
+        if (e->graph_gen != e->gen || e->graph_pin != pin || e->graph_mid_pin != mid_pin) {
+            drop_graphs(e);
+            for (int w = 0; w < 3; w++) {
+                HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+                if (w == 0) issue_open<T>(e, sp);
+                else issue_step<T>(e, sp, w == 2);
+                hipGraph_t g = nullptr;
+                HIP_TRY(hipStreamEndCapture(e->stream, &g));
+                const hipError_t err = hipGraphInstantiate(&e->graph[w], g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIP_TRY(err);
+            }
+            e->graph_gen = e->gen;
+            e->graph_pin = pin;
+            e->graph_mid_pin = mid_pin;
+        }
+        HIP_TRY(hipGraphLaunch(e->graph[0], e->stream));
+        for (int s = 0; s < nt - 1; s++) HIP_TRY(hipGraphLaunch(e->graph[1], e->stream));
+        HIP_TRY(hipGraphLaunch(e->graph[2], e->stream));
+        e->it = e->it + (unsigned)nt;
+    } else {
+        issue_open<T>(e, sp);
+        for (int s = 0; s < nt; s++) {
+            issue_step<T>(e, sp, s == nt - 1);
+            e->it = e->it + 1;
+        }
     }
     HIP_TRY(hipGetLastError());
     if (ite && mu) {
@@ -532,6 +575,7 @@ int get_psi_t(gpe_engine_s* e, cuCplx* psi, size_t inst = 0) {
 }
 
 void free_all(gpe_engine_s* e) {
+    drop_graphs(e);
     void* ptrs[] = {e->A, e->B, e->C, e->Vb, e->Vq, e->varr, e->tw[0], e->tw[1], e->tw[2], e->twlo[0], e->twlo[1],
                     e->twlo[2], e->Ktab, e->ktab, e->Khl, e->vtab,
                     e->k64, e->partials, e->scal, e->rout, e->phase, e->wind, e->adj_ck, e->adj_seg,
@@ -674,6 +718,7 @@ int gpe_engine_destroy(gpe_engine_t e) {
 
 int gpe_engine_change_alpha_beta(gpe_engine_t e, double alpha, double beta) {
     if (!e) return GPE_ERR_NO_ENGINE;
+    e->gen++;
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->alpha = alpha;
     e->beta = beta;
@@ -690,6 +735,7 @@ int gpe_engine_set_time(gpe_engine_t e, double t0) {
 
 int gpe_engine_set_user_params(gpe_engine_t e, int size, const double* params) {
     if (!e) return GPE_ERR_NO_ENGINE;
+    e->gen++;
     if (size > MAX_USER_PARAMS || size > kMaxUserParams) return GPE_ERR_TOO_MANY_PARAMS;
     if (size < 0 || (size > 0 && !params)) return GPE_ERR_ARG;
     for (int i = 0; i < kMaxUserParams; i++) e->up[i] = (i < size) ? params[i] : 0.0;
@@ -699,6 +745,7 @@ int gpe_engine_set_user_params(gpe_engine_t e, int size, const double* params) {
 // orginal/gpe_engine.cu:637-662
 int gpe_engine_set_quantum_friction_coeff(gpe_engine_t e, double qfcoeff) {
     if (!e) return GPE_ERR_NO_ENGINE;
+    e->gen++;
     HIP_TRY(hipStreamSynchronize(e->stream));
     if (qfcoeff != 0.0) {
         e->qf = qfcoeff / (e->gamma * (double)e->n);
@@ -718,6 +765,7 @@ int gpe_engine_set_quantum_friction_coeff(gpe_engine_t e, double qfcoeff) {
 // dev/gpe_engine.cu:465-476
 int gpe_engine_set_vortex(gpe_engine_t e, double x0, double y0, int Q) {
     if (!e) return GPE_ERR_NO_ENGINE;
+    e->gen++;
     e->vx0 = x0;
     e->vy0 = y0;
     e->vQ = Q;
@@ -727,6 +775,7 @@ int gpe_engine_set_vortex(gpe_engine_t e, double x0, double y0, int Q) {
 
 int gpe_engine_set_external_potential(gpe_engine_t e, const double* v) {
     if (!e) return GPE_ERR_NO_ENGINE;
+    e->gen++;
     if (!v) return GPE_ERR_ARG;
     const size_t es = esize(e);
     // one copy per lattice of a batch: the kernels index the array by the global offset
@@ -836,6 +885,7 @@ int gpe_engine_get_currents(gpe_engine_t e, double* t, double* jx, double* jy, d
 // dev/gpe_engine.cu:552-582
 int gpe_engine_get_phase(gpe_engine_t e, double* h_phase) {
     if (!e) return GPE_ERR_NO_ENGINE;
+    e->gen++;
     if (!e->phase) HIP_TRY(hipMalloc(&e->phase, e->n * sizeof(double)));
     if (e->precision == GPE_PRECISION_C64)
         launch_phase<float>((const cplx<float>*)e->A, (unsigned)e->n, e->phase, e->stream);
@@ -850,6 +900,7 @@ int gpe_engine_get_phase(gpe_engine_t e, double* h_phase) {
 
 int gpe_engine_set_phase(gpe_engine_t e, const double* h_phase) {
     if (!e) return GPE_ERR_NO_ENGINE;
+    e->gen++;
     if (!h_phase) return GPE_ERR_ARG;
     if (!e->phase) HIP_TRY(hipMalloc(&e->phase, e->n * sizeof(double)));
     HIP_TRY(hipMemcpyAsync(e->phase, h_phase, e->n * sizeof(double), hipMemcpyHostToDevice, e->stream));
@@ -1510,12 +1561,21 @@ extern "C" {
 
 int gpe_engine_autotune(gpe_engine_t e, int steps, int* variant) {
     if (!e) return GPE_ERR_NO_ENGINE;
+    e->gen++;
     HIP_TRY(hipStreamSynchronize(e->stream));
     return DISPATCH(e, autotune_t, e, steps, variant);
 }
 
+int gpe_engine_set_graph(gpe_engine_t e, int on) {
+    if (!e) return GPE_ERR_NO_ENGINE;
+    e->use_graph = (on != 0);
+    if (!e->use_graph) drop_graphs(e);
+    return 0;
+}
+
 int gpe_engine_set_variant(gpe_engine_t e, int variant) {
     if (!e) return GPE_ERR_NO_ENGINE;
+    e->gen++;
     if (variant < 0 || variant > 3) return GPE_ERR_ARG;
     e->variant = variant;
     return 0;

@@ -86,6 +86,7 @@ def lib():
             "gpe_engine_adjoint_forward": [_p, _i, _p, _i, _dp],
             "gpe_engine_adjoint_backward": [_p, _p],
             "gpe_engine_set_variant": [_p, _i],
+            "gpe_engine_set_graph": [_p, _i],
             "gpe_ensemble_create": [ctypes.POINTER(_p), _i, _p, _i, _i, _i, _i, _d, _d, _d, _d],
             "gpe_ensemble_destroy": [_p],
             "gpe_ensemble_set_user_params": [_p, _i, _p],
@@ -330,6 +331,9 @@ class Engine:
         v = ctypes.c_int()
         _check(lib().gpe_engine_autotune(self._h, int(steps), ctypes.byref(v)), "gpe_engine_autotune")
         return v.value
+
+    def set_graph(self, on):
+        _check(lib().gpe_engine_set_graph(self._h, 1 if on else 0), "gpe_engine_set_graph")
 
     def set_variant(self, variant):
         _check(lib().gpe_engine_set_variant(self._h, int(variant)), "gpe_engine_set_variant")
