@@ -253,9 +253,12 @@ void zwave_kernel(const PassArgs<T> a) {
                               MM == MID_QF_VSTEP1 || MM == MID_OVERLAP_RE;
     const T* rs = real_side(a);
     const bool qf = (a.flags & F_QF) != 0;
-    // the fast chunk forms: real-time ops without imprint
+    // the fast chunk forms: ops without imprint (complex128 keeps the per-point
+    // form of the predictor end and the QF step, whose chunk forms spill at 256)
+    constexpr bool CHUNK_PQ = sizeof(T) == 4;
     const bool fast = (MM == MID_VSTEP2) || (MM == MID_VSTEP3 && !(a.flags & F_IMPRINT_NEXT)) ||
-                      (LM == LOAD_PSI0 && !(a.flags & F_IMPRINT));
+                      (LM == LOAD_PSI0 && !(a.flags & F_IMPRINT)) || (LM == LOAD_PART2 && !(a.flags & F_IMPRINT_MID)) ||
+                      (CHUNK_PQ && (MM == MID_PART1 || MM == MID_QF_VSTEP1));
     cplx<T> ax[2][CH];
     T rx[2][CH];
     auto fetch = [&](int q, int slot) __attribute__((always_inline)) {
@@ -325,6 +328,59 @@ void zwave_kernel(const PassArgs<T> a) {
                 if (ok) {
 #pragma unroll
                     for (int j = 0; j < CH; j++) a.vbuf[off_of(q * CH + j)] = lv[j];
+                }
+                mul_exp_chunk(p0, lv, a.ph.c.x, a.ph.c.y);
+#pragma unroll
+                for (int j = 0; j < CH; j++) u[j] = p0[j];
+            } else if constexpr (LM == LOAD_PART2) {
+                // corrector of imaginary time (orginal/gpe_engine.cu:817-868): u is phi3
+                if (a.flags & F_SCALE) {
+                    const T f = (T)a.scal[1];
+#pragma unroll
+                    for (int j = 0; j < CH; j++) u[j] = cscale(u[j], f);
+                }
+                cplx<T> p0[CH];
+                T V0[CH];
+#pragma unroll
+                for (int j = 0; j < CH; j++) p0[j] = ax[slot][j];
+                dedfdn_chunk(a.ph, p0, gv);
+#pragma unroll
+                for (int j = 0; j < CH; j++) V0[j] = vx[j] + gv[j] + (qf ? rx[slot][j] : (T)0);
+                dedfdn_chunk(a.ph, u, gv);
+#pragma unroll
+                for (int j = 0; j < CH; j++) lv[j] = (T)0.5 * (V0[j] + vx[j] + gv[j] + (qf ? rx[slot][j] : (T)0));
+                if (ok) {
+#pragma unroll
+                    for (int j = 0; j < CH; j++) a.vbuf[off_of(q * CH + j)] = lv[j];
+                }
+                mul_exp_chunk(p0, lv, a.ph.c.x, a.ph.c.y);
+#pragma unroll
+                for (int j = 0; j < CH; j++) u[j] = p0[j];
+            } else if constexpr (CHUNK_PQ && MM == MID_PART1) {
+                // predictor end with the density sum (orginal/gpe_engine.cu:803-815)
+                cplx<T> p0[CH];
+#pragma unroll
+                for (int j = 0; j < CH; j++) p0[j] = ax[slot][j];
+                dedfdn_chunk(a.ph, p0, gv);
+#pragma unroll
+                for (int j = 0; j < CH; j++) lv[j] = vx[j] + gv[j] + (qf ? rx[slot][j] : (T)0);
+                mul_exp_chunk(u, lv, a.ph.c.x, a.ph.c.y);
+                if (ok) {
+#pragma unroll
+                    for (int j = 0; j < CH; j++) acc[0] += dens64((double)a.ph.gamma, (double)u[j].x, (double)u[j].y);
+                }
+            } else if constexpr (CHUNK_PQ && MM == MID_QF_VSTEP1) {
+                // quantum-friction potential from F^-1[qf' k^2 F psi0] (orginal 911-922), then Vstep1_qf
+                cplx<T> p0[CH];
+#pragma unroll
+                for (int j = 0; j < CH; j++) p0[j] = ax[slot][j];
+                dedfdn_chunk(a.ph, p0, gv);
+#pragma unroll
+                for (int j = 0; j < CH; j++) {
+                    const T r0 = density(a.ph, p0[j]);
+                    const T qv = (p0[j].x * u[j].y + (-p0[j].y) * u[j].x) / (r0 + (T)1.0e-12);
+                    if (ok) a.vq[off_of(q * CH + j)] = qv;
+                    lv[j] = vx[j] + gv[j] + qv;
                 }
                 mul_exp_chunk(p0, lv, a.ph.c.x, a.ph.c.y);
 #pragma unroll
