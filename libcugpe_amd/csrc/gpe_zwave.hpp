@@ -171,13 +171,14 @@ __device__ __forceinline__ void mul_exp_chunk(cplx<double> (&v)[CH], const doubl
 #endif
 // resident waves per SIMD the compiler must allow (VGPR budget 512 / waves):
 // complex64 4, or ZW_WAVES_PHYS for the passes with physics between transforms
-template <typename T, int SHAPE, int MM>
+// and for the imaginary-time corrector
+template <typename T, int SHAPE, int LM, int MM>
 constexpr int zw_waves() {
-    return sizeof(T) == 4 ? ((SHAPE == SHAPE_IF && MM != MID_NONE) ? ZW_WAVES_PHYS : 4) : 2;
+    return sizeof(T) == 4 ? (((SHAPE == SHAPE_IF && MM != MID_NONE) || LM == LOAD_PART2) ? ZW_WAVES_PHYS : 4) : 2;
 }
 
 template <typename T, int N, int SHAPE, int LM, int MM>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(zw_waves<T, SHAPE, MM>())))
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(zw_waves<T, SHAPE, LM, MM>())))
 void zwave_kernel(const PassArgs<T> a) {
     using Z = ZW<T, N>;
     constexpr int A = Z::A, B = Z::B;
@@ -238,7 +239,7 @@ void zwave_kernel(const PassArgs<T> a) {
     // computed, and those of chunk 0 before the first transform, so their
     // latency overlaps arithmetic.
 #ifndef ZW_CH
-#define ZW_CH 4
+#define ZW_CH 8
 #endif
 #ifndef ZW_PREFETCH
 #define ZW_PREFETCH 1
