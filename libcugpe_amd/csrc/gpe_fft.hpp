@@ -399,6 +399,28 @@ __device__ __forceinline__ void sincos_small(float x, float* s, float* c) {
     }
 }
 
+// complex128: sin and cos of |x| <= 1/4 from their Taylor series through
+// x^13 and x^12 (the first omitted terms are below 1e-17 relative)
+__device__ __forceinline__ void sincos_taylor(double x, double* s, double* c) {
+    const double t = x * x;
+    double ps = fma(t, -1.0 / 6227020800.0, 1.0 / 39916800.0);
+    ps = fma(t, ps, -1.0 / 362880.0);
+    ps = fma(t, ps, 1.0 / 5040.0);
+    ps = fma(t, ps, -1.0 / 120.0);
+    ps = fma(t, ps, 1.0 / 6.0);
+    *s = fma(-x * t, ps, x);
+    double pc = fma(t, 1.0 / 479001600.0, -1.0 / 3628800.0);
+    pc = fma(t, pc, 1.0 / 40320.0);
+    pc = fma(t, pc, -1.0 / 720.0);
+    pc = fma(t, pc, 1.0 / 24.0);
+    pc = fma(t, pc, -0.5);
+    *c = fma(t, pc, 1.0);
+}
+__device__ __forceinline__ void sincos_small(double x, double* s, double* c) {
+    if (fabs(x) <= 0.25) sincos_taylor(x, s, c);
+    else sincos(x, s, c);
+}
+
 // v * exp(re + i im).  In complex64 the rounded direction (c, s) misses unit
 // modulus by e = c^2 + s^2 - 1 ~ 1e-8, the same way every step; e is formed
 // without cancellation by two FMAs and the product is scaled by 1 + h,
@@ -409,7 +431,14 @@ __device__ __forceinline__ void sincos_small(float x, float* s, float* c) {
 // whose inner terms are small (s ~ |cV|) and whose dropped part h (r - v) is
 // O(h |cV|).  The factor is then a rotation to ~1e-15 up to unbiased roundings.
 __device__ __forceinline__ cplx<double> mul_exp(cplx<double> v, double re, double im) {
-    return cmul(v, cexp_(re, im));
+    double s, c;
+    sincos_small(im, &s, &c);
+    cplx<double> r = mk(v.x * c - v.y * s, v.x * s + v.y * c);
+    if (re != 0.0) {  // real time: re == 0 for every point
+        const double g = exp(re);
+        r = mk(r.x * g, r.y * g);
+    }
+    return r;
 }
 __device__ __forceinline__ cplx<float> mul_exp(cplx<float> v, float re, float im) {
     float s, c;

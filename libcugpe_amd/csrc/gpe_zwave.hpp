@@ -159,10 +159,35 @@ __device__ __forceinline__ void mul_exp_chunk(cplx<float> (&v)[CH], const float 
         }
     }
 }
+// complex128: the small-angle series for the whole chunk, sincos for the
+// lanes past 1/4, exp only when the step has a real part (imaginary or
+// complex time)
 template <int CH>
 __device__ __forceinline__ void mul_exp_chunk(cplx<double> (&v)[CH], const double (&lv)[CH], double cx, double cy) {
+    double s[CH], c[CH];
+    bool big = false;
 #pragma unroll
-    for (int j = 0; j < CH; j++) v[j] = mul_exp(v[j], cx * lv[j], cy * lv[j]);
+    for (int j = 0; j < CH; j++) {
+        const double x = cy * lv[j];
+        sincos_taylor(x, &s[j], &c[j]);
+        big = big || (fabs(x) > 0.25);
+    }
+    if (big) {
+#pragma unroll
+        for (int j = 0; j < CH; j++) {
+            const double x = cy * lv[j];
+            if (fabs(x) > 0.25) sincos(x, &s[j], &c[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < CH; j++) v[j] = mk(v[j].x * c[j] - v[j].y * s[j], v[j].x * s[j] + v[j].y * c[j]);
+    if (cx != 0.0) {
+#pragma unroll
+        for (int j = 0; j < CH; j++) {
+            const double gm = exp(cx * lv[j]);
+            v[j] = mk(v[j].x * gm, v[j].y * gm);
+        }
+    }
 }
 
 // SHAPE: F, I or IF; LM: load op before the first transform; MM: op after it
@@ -244,8 +269,8 @@ void zwave_kernel(const PassArgs<T> a) {
 #ifndef ZW_PREFETCH
 #define ZW_PREFETCH 1
 #endif
-    // complex128 holds twice the registers per point: chunks of two
-    constexpr int CHT = sizeof(T) == 4 ? ZW_CH : 2;
+    // complex128 holds twice the registers per point: chunks of four
+    constexpr int CHT = sizeof(T) == 4 ? ZW_CH : 4;
     constexpr int CH = (B >= CHT) ? CHT : B;
     constexpr int NCH = B / CH;
     constexpr bool PRE = (LM != LOAD_PLAIN);   // physics before the first transform
@@ -254,12 +279,10 @@ void zwave_kernel(const PassArgs<T> a) {
                               MM == MID_QF_VSTEP1 || MM == MID_OVERLAP_RE;
     const T* rs = real_side(a);
     const bool qf = (a.flags & F_QF) != 0;
-    // the fast chunk forms: ops without imprint (complex128 keeps the per-point
-    // form of the predictor end and the QF step, whose chunk forms spill at 256)
-    constexpr bool CHUNK_PQ = sizeof(T) == 4;
+    // the fast chunk forms: ops without imprint
     const bool fast = (MM == MID_VSTEP2) || (MM == MID_VSTEP3 && !(a.flags & F_IMPRINT_NEXT)) ||
                       (LM == LOAD_PSI0 && !(a.flags & F_IMPRINT)) || (LM == LOAD_PART2 && !(a.flags & F_IMPRINT_MID)) ||
-                      (CHUNK_PQ && (MM == MID_PART1 || MM == MID_QF_VSTEP1));
+                      MM == MID_PART1 || MM == MID_QF_VSTEP1;
     cplx<T> ax[2][CH];
     T rx[2][CH];
     auto fetch = [&](int q, int slot) __attribute__((always_inline)) {
@@ -357,7 +380,7 @@ void zwave_kernel(const PassArgs<T> a) {
                 mul_exp_chunk(p0, lv, a.ph.c.x, a.ph.c.y);
 #pragma unroll
                 for (int j = 0; j < CH; j++) u[j] = p0[j];
-            } else if constexpr (CHUNK_PQ && MM == MID_PART1) {
+            } else if constexpr (MM == MID_PART1) {
                 // predictor end with the density sum (orginal/gpe_engine.cu:803-815)
                 cplx<T> p0[CH];
 #pragma unroll
@@ -370,7 +393,7 @@ void zwave_kernel(const PassArgs<T> a) {
 #pragma unroll
                     for (int j = 0; j < CH; j++) acc[0] += dens64((double)a.ph.gamma, (double)u[j].x, (double)u[j].y);
                 }
-            } else if constexpr (CHUNK_PQ && MM == MID_QF_VSTEP1) {
+            } else if constexpr (MM == MID_QF_VSTEP1) {
                 // quantum-friction potential from F^-1[qf' k^2 F psi0] (orginal 911-922), then Vstep1_qf
                 cplx<T> p0[CH];
 #pragma unroll
