@@ -400,7 +400,8 @@ __device__ __forceinline__ void sincos_small(float x, float* s, float* c) {
 }
 
 // complex128: sin and cos of |x| <= 1/4 from their Taylor series through
-// x^13 and x^12 (the first omitted terms are below 1e-17 relative)
+// x^13 and x^12 (the first omitted terms are below 1e-17 relative); the
+// chunked physics of the z passes uses it, the per-point ops keep sincos
 __device__ __forceinline__ void sincos_taylor(double x, double* s, double* c) {
     const double t = x * x;
     double ps = fma(t, -1.0 / 6227020800.0, 1.0 / 39916800.0);
@@ -416,10 +417,6 @@ __device__ __forceinline__ void sincos_taylor(double x, double* s, double* c) {
     pc = fma(t, pc, -0.5);
     *c = fma(t, pc, 1.0);
 }
-__device__ __forceinline__ void sincos_small(double x, double* s, double* c) {
-    if (fabs(x) <= 0.25) sincos_taylor(x, s, c);
-    else sincos(x, s, c);
-}
 
 // v * exp(re + i im).  In complex64 the rounded direction (c, s) misses unit
 // modulus by e = c^2 + s^2 - 1 ~ 1e-8, the same way every step; e is formed
@@ -431,14 +428,7 @@ __device__ __forceinline__ void sincos_small(double x, double* s, double* c) {
 // whose inner terms are small (s ~ |cV|) and whose dropped part h (r - v) is
 // O(h |cV|).  The factor is then a rotation to ~1e-15 up to unbiased roundings.
 __device__ __forceinline__ cplx<double> mul_exp(cplx<double> v, double re, double im) {
-    double s, c;
-    sincos_small(im, &s, &c);
-    cplx<double> r = mk(v.x * c - v.y * s, v.x * s + v.y * c);
-    if (re != 0.0) {  // real time: re == 0 for every point
-        const double g = exp(re);
-        r = mk(r.x * g, r.y * g);
-    }
-    return r;
+    return cmul(v, cexp_(re, im));
 }
 __device__ __forceinline__ cplx<float> mul_exp(cplx<float> v, float re, float im) {
     float s, c;
