@@ -332,7 +332,7 @@ def splitmix64_uniform(seed, n):
     return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
 
 
-def config1_bench(E):
+def ite_ground_state_bench(E):
     """Imaginary-time ground state of a trapped contact BEC, 32^3 complex128:
     box L = 20 (oscillator units, dx = 0.625), V = r^2/2, N = 1e4, g = 1.25
     (read as 1.0 N / L^3), Gaussian start of width 2 times (1 + 0.05 U) with U
@@ -382,7 +382,7 @@ def config1_bench(E):
     }
 
 
-def config2_bench(E, steps, oracle_steps):
+def ufg_vortex_qf_bench(E, steps, oracle_steps):
     """Real-time unitary Fermi gas (dimers, xi = 0.37) with quantum friction
     0.1 in an anisotropic trap omega = (1, 1, 0.5), 128^3, box L = 32 (dx =
     0.25), dt = 5e-4, from a seeded Thomas-Fermi density with one vortex line
@@ -465,15 +465,15 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--no-configs", action="store_true")
-    ap.add_argument("--config2-steps", type=int, default=10000)
-    ap.add_argument("--config2-oracle-steps", type=int, default=40)
+    ap.add_argument("--no-extra-workloads", action="store_true")
+    ap.add_argument("--ufg-steps", type=int, default=10000)
+    ap.add_argument("--ufg-oracle-steps", type=int, default=40)
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-oracle-steps", type=int, default=200)
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py: this engine runs on one GPU; a %d-GPU run is not part of this build "
-              "(the sharded design is described in DESIGN.md)" % a.gpus, file=sys.stderr)
+        print("bench.py: this engine runs on one GPU; a %d-GPU run is not supported "
+              "(one lattice is not sharded across GPUs)" % a.gpus, file=sys.stderr)
         return 2
 
     from libcugpe_amd import engine as E
@@ -584,12 +584,12 @@ def main():
         print("[bench] batched ensemble (%d steps)" % a.ensemble_steps, file=sys.stderr, flush=True)
         result["ensemble"] = ensemble_bench(E, a.ensemble_steps, 50, 8, 100)
 
-    if not a.no_configs:
-        print("[bench] config 1: 32^3 complex128 ground state", file=sys.stderr, flush=True)
-        result["config1"] = config1_bench(E)
-        print("[bench] config 2: 128^3 UFG with quantum friction (%d steps)" % a.config2_steps, file=sys.stderr,
+    if not a.no_extra_workloads:
+        print("[bench] 32^3 complex128 ground state", file=sys.stderr, flush=True)
+        result["ite_ground_state_32"] = ite_ground_state_bench(E)
+        print("[bench] 128^3 UFG with quantum friction (%d steps)" % a.ufg_steps, file=sys.stderr,
               flush=True)
-        result["config2"] = config2_bench(E, a.config2_steps, a.config2_oracle_steps)
+        result["ufg_vortex_qf_128"] = ufg_vortex_qf_bench(E, a.ufg_steps, a.ufg_oracle_steps)
 
     if not a.no_adjoint:
         print("[bench] adjoint gradient (T=%d)" % a.adjoint_steps, file=sys.stderr, flush=True)
