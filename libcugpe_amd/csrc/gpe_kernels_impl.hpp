@@ -21,7 +21,7 @@ unsigned launch_fft_pass_impl(int N, const PassArgs<T>& a, hipStream_t s) {
 template <typename T, int NN>
 constexpr int min_tile() {
     constexpr int w = full_tile_w<T, NN, true>();
-    if constexpr (zw_lanes(NN) != 0) return ZW<T, NN>::PENCILS < w ? ZW<T, NN>::PENCILS : w;
+    if constexpr (zw_lanes<T>(NN) != 0) return ZW<T, NN>::PENCILS < w ? ZW<T, NN>::PENCILS : w;
     return w;
 }
 

@@ -24,7 +24,10 @@
 
 namespace gpe {
 
-// lanes per pencil; 0: the length runs in the LDS-tile kernels
+// lanes per pencil; 0: the length runs in the LDS-tile kernels.  512 = 16
+// lanes x 32 points only in complex64 (complex128 would hold 64 registers of
+// data per lane and twice the exchange rows)
+template <typename T>
 constexpr int zw_lanes(int N) {
     switch (N) {
     case 8: return 2;
@@ -33,19 +36,22 @@ constexpr int zw_lanes(int N) {
     case 64: return 8;
     case 128: return 8;
     case 256: return 16;
+    case 512: return sizeof(T) == 4 ? 16 : 0;
     default: return 0;
     }
 }
 
 template <typename T, int N>
 struct ZW {
-    static constexpr int A = zw_lanes(N);
+    static constexpr int A = zw_lanes<T>(N);
     static constexpr int B = N / A;
     static constexpr int PW = 64 / A;             // pencils per wave
     static constexpr int ROW = A + 1;             // padded exchange row
+    // exchange rows held at once: all B, or A in two phases when B = 2A > 16
+    static constexpr int XR = (B > 16) ? A : B;
     // exchange elements per pencil, = 16 mod 32 so that consecutive pencils of
     // a wave start half a bank cycle apart (8-byte elements, 64 banks)
-    static constexpr int GS = B * ROW + ((16 - (B * ROW) % 32) + 32) % 32;
+    static constexpr int GS = XR * ROW + ((16 - (XR * ROW) % 32) + 32) % 32;
     static constexpr int WAVE = PW * GS;          // exchange elements per wave
     static constexpr int WAVES = kThreads / 64;
     static constexpr int PENCILS = PW * WAVES;    // pencils per workgroup
@@ -81,17 +87,41 @@ __device__ __forceinline__ void zw_fft(cplx<T> (&v)[B], cplx<T>* X, const TwE<T>
     dft<B, S>(v);
 #pragma unroll
     for (int kb = 1; kb < B; kb++) v[kb] = tw_mul<S>(v[kb], tw[kb * A + l]);  // w_N^(l kb)
+    if constexpr (B > 16) {
+        // two phases through A rows: columns kb < A, then kb >= A.  The second
+        // half is in LDS before the first phase's outputs overwrite its registers.
+        static_assert(B == 2 * A, "two-phase exchange needs B = 2A");
 #pragma unroll
-    for (int kb = 0; kb < B; kb++) X[kb * ROW + l] = v[kb];
-    wave_lds_sync();
-#pragma unroll
-    for (int m = 0; m < B / A; m++) {
+        for (int kb = 0; kb < A; kb++) X[kb * ROW + l] = v[kb];
+        wave_lds_sync();
         cplx<T> u[A];
 #pragma unroll
-        for (int q = 0; q < A; q++) u[q] = X[(l + A * m) * ROW + q];
+        for (int q = 0; q < A; q++) u[q] = X[l * ROW + q];
+        wave_lds_sync();
+#pragma unroll
+        for (int kb = 0; kb < A; kb++) X[kb * ROW + l] = v[A + kb];
+        wave_lds_sync();
         dft<A, S>(u);
 #pragma unroll
-        for (int ka = 0; ka < A; ka++) v[m + (B / A) * ka] = u[ka];
+        for (int ka = 0; ka < A; ka++) v[2 * ka] = u[ka];
+#pragma unroll
+        for (int q = 0; q < A; q++) u[q] = X[l * ROW + q];
+        dft<A, S>(u);
+#pragma unroll
+        for (int ka = 0; ka < A; ka++) v[1 + 2 * ka] = u[ka];
+    } else {
+#pragma unroll
+        for (int kb = 0; kb < B; kb++) X[kb * ROW + l] = v[kb];
+        wave_lds_sync();
+#pragma unroll
+        for (int m = 0; m < B / A; m++) {
+            cplx<T> u[A];
+#pragma unroll
+            for (int q = 0; q < A; q++) u[q] = X[(l + A * m) * ROW + q];
+            dft<A, S>(u);
+#pragma unroll
+            for (int ka = 0; ka < A; ka++) v[m + (B / A) * ka] = u[ka];
+        }
     }
     wave_lds_sync();
 }
@@ -197,13 +227,18 @@ __device__ __forceinline__ void mul_exp_chunk(cplx<double> (&v)[CH], const doubl
 // resident waves per SIMD the compiler must allow (VGPR budget 512 / waves):
 // complex64 4, or ZW_WAVES_PHYS for the passes with physics between transforms
 // and for the imaginary-time corrector
-template <typename T, int SHAPE, int LM, int MM>
+// (512 points: 32 registers of data per lane and three workgroups per CU by
+// LDS; 2 waves with physics, 3 without)
+template <typename T, int N, int SHAPE, int LM, int MM>
 constexpr int zw_waves() {
-    return sizeof(T) == 4 ? (((SHAPE == SHAPE_IF && MM != MID_NONE) || LM == LOAD_PART2) ? ZW_WAVES_PHYS : 4) : 2;
+    constexpr bool phys = (SHAPE == SHAPE_IF && MM != MID_NONE) || LM == LOAD_PART2;
+    if constexpr (sizeof(T) != 4) return 2;
+    else if constexpr (N > 256) return (phys || MM != MID_NONE || LM != LOAD_PLAIN) ? 2 : 3;
+    else return phys ? ZW_WAVES_PHYS : 4;
 }
 
 template <typename T, int N, int SHAPE, int LM, int MM>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(zw_waves<T, SHAPE, LM, MM>())))
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(zw_waves<T, N, SHAPE, LM, MM>())))
 void zwave_kernel(const PassArgs<T> a) {
     using Z = ZW<T, N>;
     constexpr int A = Z::A, B = Z::B;
@@ -269,8 +304,9 @@ void zwave_kernel(const PassArgs<T> a) {
 #ifndef ZW_PREFETCH
 #define ZW_PREFETCH 1
 #endif
-    // complex128 holds twice the registers per point: chunks of four
-    constexpr int CHT = sizeof(T) == 4 ? ZW_CH : 4;
+    // complex128 holds twice the registers per point, 512-point pencils twice
+    // the points per lane: chunks of four
+    constexpr int CHT = (sizeof(T) == 4 && N <= 256) ? ZW_CH : 4;
     constexpr int CH = (B >= CHT) ? CHT : B;
     constexpr int NCH = B / CH;
     constexpr bool PRE = (LM != LOAD_PLAIN);   // physics before the first transform
@@ -486,7 +522,7 @@ void zwave_kernel(const PassArgs<T> a) {
 // the LDS-tile kernels
 template <typename T, int N>
 unsigned launch_zwave(const PassArgs<T>& a, hipStream_t s) {
-    if constexpr (zw_lanes(N) == 0) {
+    if constexpr (zw_lanes<T>(N) == 0) {
         return 0;
     } else {
         if (a.axis != 2 || a.nred > 2) return 0;

@@ -40,7 +40,8 @@ def pair(oracle, shape, alpha, beta, dt, npart, interactions, vext, params, prec
     return g, o
 
 
-@pytest.mark.parametrize("shape", [(32, 32, 32), (256, 32, 32), (24, 16, 12), (48, 8, 64), (16, 96, 8)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (256, 32, 32), (24, 16, 12), (48, 8, 64), (16, 96, 8), (8, 8, 512),
+                                   (512, 8, 8)])
 @pytest.mark.parametrize("prec", [E.C128, E.C64])
 def test_fft3d_matches_numpy(shape, prec):
     say("fft3d", shape, prec)
@@ -76,6 +77,32 @@ def test_plane_wave_exact_phase(prec, tol):
     assert abs(t - 4 * dt) < 1e-15
     assert rel_err(out, psi * np.exp(4j * ph)) < tol
     g.close()
+
+
+@pytest.mark.parametrize("mode", ["rte", "ite", "qf"])
+def test_512_point_pencils_complex64_vs_complex128(mode):
+    """z pencils of 512 points: complex64 runs them in the wave kernels (16
+    lanes x 32 points, two-phase exchange), complex128 in the LDS-tile
+    kernels, so the two precisions check each other's code paths.  Bound:
+    2e-5 of max |psi| after 12 steps (complex64 rounding)."""
+    say("512-point pencils", mode)
+    shape = (8, 12, 512)
+    psi = seeded_state(shape, 21)
+    out = {}
+    for prec in (E.C64, E.C128):
+        alpha, beta = (1.0, 0.0) if mode != "ite" else (0.0, 1.0)
+        g = E.Engine(shape, alpha, beta, 0.02, float(np.prod(shape)) * 0.1, E.CONTACT, E.COSINE, 0, prec)
+        g.set_user_params([0.05, 4.0, 0.0, 0.01])
+        if mode == "qf":
+            g.set_quantum_friction_coeff(0.1)
+        g.set_psi(psi)
+        g.normalize_psi()
+        g.evolve_qf(12)
+        out[prec] = g.get_psi()[0]
+        g.close()
+    err = rel_err(out[E.C64], out[E.C128])
+    say("512 %s: c64 vs c128 %.2e" % (mode, err))
+    assert err < 2e-5
 
 
 CASES = [
