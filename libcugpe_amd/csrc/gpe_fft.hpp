@@ -33,7 +33,10 @@ struct FftPlan {
     int r[8];
 };
 
-constexpr FftPlan make_plan(int N) {
+// c64: complex64 (512 = 32 x 16: its lean tiles are 16 columns wide, so every
+// thread owns a radix-32 or radix-16 butterfly; complex128 keeps 8 x 8 x 8
+// for its 4-column tiles)
+constexpr FftPlan make_plan(int N, bool c64) {
     FftPlan p{0, {0, 0, 0, 0, 0, 0, 0, 0}};
     switch (N) {  // hand-picked schedules for the common lengths
     case 16: p.ns = 1; p.r[0] = 16; return p;
@@ -41,7 +44,10 @@ constexpr FftPlan make_plan(int N) {
     case 64: p.ns = 2; p.r[0] = 8; p.r[1] = 8; return p;
     case 128: p.ns = 2; p.r[0] = 16; p.r[1] = 8; return p;
     case 256: p.ns = 2; p.r[0] = 16; p.r[1] = 16; return p;
-    case 512: p.ns = 3; p.r[0] = 8; p.r[1] = 8; p.r[2] = 8; return p;
+    case 512:
+        if (c64) { p.ns = 2; p.r[0] = 32; p.r[1] = 16; }
+        else { p.ns = 3; p.r[0] = 8; p.r[1] = 8; p.r[2] = 8; }
+        return p;
     default: break;
     }
     int n = N;
@@ -60,15 +66,16 @@ constexpr int plan_span(const FftPlan& p, int s) {
     return v;
 }
 
-constexpr bool plan_ok(int N) {
-    FftPlan p = make_plan(N);
+constexpr bool plan_ok(int N, bool c64) {
+    FftPlan p = make_plan(N, c64);
     return p.ns > 0 && plan_span(p, p.ns) == N;
 }
 
 // tile width (pencils per workgroup), a power of two
 template <typename T, int N>
 constexpr int tile_w() {
-    int base = (sizeof(T) == 4) ? 4096 : 2048;
+    // complex64 512: 16 columns (128-byte rows) rather than 8
+    int base = (sizeof(T) == 4) ? (N >= 512 ? 8192 : 4096) : 2048;
     int w = base / N;
     if (w < 1) w = 1;
     int p2 = 1;
@@ -331,7 +338,7 @@ __device__ __forceinline__ void fft_stage(cplx<T>* lds, const TwE<T>* tw) {
 
 template <typename T, int N, int W, bool CONTIG, int S, int STAGE>
 __device__ __forceinline__ void fft_stages(cplx<T>* lds, const TwE<T>* tw) {
-    constexpr FftPlan P = make_plan(N);
+    constexpr FftPlan P = make_plan(N, sizeof(T) == 4);
     if constexpr (STAGE < P.ns) {
         fft_stage<T, N, W, CONTIG, S, P.r[STAGE], plan_span(P, STAGE)>(lds, tw);
         fft_stages<T, N, W, CONTIG, S, STAGE + 1>(lds, tw);
@@ -737,8 +744,8 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // transform, so the mid op and the hand-over stay in registers.  The final
 // registers go straight to global memory.
 
-constexpr FftPlan plan_of(int N, bool rev) {
-    FftPlan p = make_plan(N);
+constexpr FftPlan plan_of(int N, bool rev, bool c64) {
+    FftPlan p = make_plan(N, c64);
     if (rev)
         for (int i = 0; i < p.ns / 2; i++) {
             int t = p.r[i];
@@ -865,7 +872,7 @@ __device__ __forceinline__ void first_stage_regs(cplx<T> (&v)[PER][R], cplx<T>* 
 template <typename T, int N, int W, bool CONTIG, int S, bool REV, int STAGE, int LAST>
 __device__ __forceinline__ void lds_stages(cplx<T>* lds, const TwE<T>* tw) {
     if constexpr (STAGE < LAST) {
-        constexpr FftPlan P = plan_of(N, REV);
+        constexpr FftPlan P = plan_of(N, REV, sizeof(T) == 4);
         fft_stage<T, N, W, CONTIG, S, P.r[STAGE], plan_span(P, STAGE)>(lds, tw);
         lds_stages<T, N, W, CONTIG, S, REV, STAGE + 1, LAST>(lds, tw);
     }
@@ -974,7 +981,7 @@ constexpr int lean_w() {
 template <typename T, int N, bool CONTIG, int S1, int S2, int WD, int MK, bool CJ, int AX>
 __device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, const TwE<T>* tw) {
     constexpr int W = lean_w<T, N, WD>();
-    constexpr FftPlan P = plan_of(N, false);
+    constexpr FftPlan P = plan_of(N, false, sizeof(T) == 4);
     constexpr int NSG = P.ns;
     constexpr int RL = P.r[NSG - 1];
     constexpr int PERL = (W * (N / RL) + kThreads - 1) / kThreads;
@@ -996,7 +1003,7 @@ __device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, co
 #pragma unroll
             for (int q = 0; q < PERL; q++) dft<RL, S2>(v[q]);
         } else {
-            constexpr FftPlan Q = plan_of(N, true);
+            constexpr FftPlan Q = plan_of(N, true, sizeof(T) == 4);
             static_assert(Q.r[0] == RL, "reversed plan must start with the last radix");
             constexpr int RF = Q.r[NSG - 1];
             constexpr int PERF = (W * (N / RF) + kThreads - 1) / kThreads;
@@ -1083,7 +1090,7 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
     constexpr int W = FullLds<T, N, CONTIG>::W;
     constexpr int TILE = W * N;
     constexpr int PERE = (TILE + kThreads - 1) / kThreads;
-    constexpr FftPlan P = plan_of(N, false);
+    constexpr FftPlan P = plan_of(N, false, sizeof(T) == 4);
     constexpr int NSG = P.ns;
     constexpr int RL = P.r[NSG - 1];
     constexpr int PERL = (W * (N / RL) + kThreads - 1) / kThreads;

@@ -9,7 +9,7 @@
 namespace gpe {
 template <typename T, int NN>
 unsigned launch_fft_pass_n_impl(const PassArgs<T>& a, hipStream_t s) {
-    static_assert(plan_ok(NN), "bad FFT plan");
+    static_assert(plan_ok(NN, sizeof(T) == 4), "bad FFT plan");
     // z passes: the wave kernels unless variant bit 0 asks for the LDS tiles
     if (a.axis == 2 && !(a.variant & 1)) {
         const unsigned zb = launch_zwave<T, NN>(a, s);
