@@ -12,6 +12,9 @@ bool fft_length_supported(int N);
 // pencils per workgroup for a compiled length
 template <typename T>
 int fft_tile_width(int N);
+// pencils per workgroup of the full-width lean strided passes
+template <typename T>
+int fft_lean_width(int N);
 
 // one FFT axis pass; returns the number of workgroups launched (0 on error)
 template <typename T>

@@ -37,6 +37,18 @@ int fft_tile_width_impl(int N) {
 #undef GPE_CASE
 }
 
+template <typename T>
+int fft_lean_width_impl(int N) {
+#define GPE_CASE(NN) \
+    case NN: return lean_w<T, NN, 1>();
+    switch (N) {
+        GPE_FFT_LENGTHS(GPE_CASE)
+    default:
+        return 0;
+    }
+#undef GPE_CASE
+}
+
 // ---------------------------------------------------------- reductions
 
 template <typename T>
@@ -226,6 +238,10 @@ static inline dim3 ew_grid(unsigned n) { return dim3((n + kThreads - 1) / kThrea
     template <>                                                                                                   \
     int fft_tile_width<T>(int N) {                                                                                \
         return fft_tile_width_impl<T>(N);                                                                         \
+    }                                                                                                             \
+    template <>                                                                                                   \
+    int fft_lean_width<T>(int N) {                                                                                \
+        return fft_lean_width_impl<T>(N);                                                                         \
     }                                                                                                             \
     template <>                                                                                                   \
     int launch_norm_partials<T>(const cplx<T>* A, unsigned n, double gamma, double* partials, hipStream_t s) {    \
