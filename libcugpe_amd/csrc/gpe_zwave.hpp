@@ -57,6 +57,32 @@ struct ZW {
     static constexpr int PENCILS = PW * WAVES;    // pencils per workgroup
 };
 
+// Side arrays (psi0, V1, V_qf) are written by one z pass and read back by
+// the next z pass four axis passes later; the state array in between is
+// read and written by every pass.  Their accesses are non-temporal, so they
+// do not push the state array out of the caches (measured +6% steps/s at
+// 256^3 complex64, bit-identical results).
+template <typename T>
+__device__ __forceinline__ void st_nt(cplx<T>* p, cplx<T> v) {
+    using V2 = T __attribute__((ext_vector_type(2)));
+    V2 w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<V2*>(p));
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> ld_nt(const cplx<T>* p) {
+    using V2 = T __attribute__((ext_vector_type(2)));
+    const V2 w = __builtin_nontemporal_load(reinterpret_cast<const V2*>(p));
+    return mk(w.x, w.y);
+}
+template <typename T>
+__device__ __forceinline__ void st_nt(T* p, T v) {
+    __builtin_nontemporal_store(v, p);
+}
+template <typename T>
+__device__ __forceinline__ T ld_nt(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+
 // load mode of the energy pass: plain load plus the potential-energy sums
 constexpr int LOAD_REDPOT = 3;
 
@@ -325,11 +351,11 @@ void zwave_kernel(const PassArgs<T> a) {
 #pragma unroll
         for (int j = 0; j < CH; j++) {
             const unsigned off = base + (unsigned)(A * (q * CH + j));
-            ax[slot][j] = NEED_AUX ? a.psiA[off] : mk((T)0, (T)0);
+            ax[slot][j] = NEED_AUX ? ld_nt(&a.psiA[off]) : mk((T)0, (T)0);
         }
         if (rs) {
 #pragma unroll
-            for (int j = 0; j < CH; j++) rx[slot][j] = rs[off_of(q * CH + j)];
+            for (int j = 0; j < CH; j++) rx[slot][j] = ld_nt(&rs[off_of(q * CH + j)]);
         } else {
 #pragma unroll
             for (int j = 0; j < CH; j++) rx[slot][j] = (T)0;
@@ -363,7 +389,7 @@ void zwave_kernel(const PassArgs<T> a) {
                     for (int j = 0; j < CH; j++) u[j] = cscale(u[j], f);
                     if (ok) {
 #pragma unroll
-                        for (int j = 0; j < CH; j++) a.psiA[off_of(q * CH + j)] = u[j];
+                        for (int j = 0; j < CH; j++) st_nt(&a.psiA[off_of(q * CH + j)], u[j]);
                     }
                 }
                 if (a.flags & F_VSTEP1) {
@@ -387,7 +413,7 @@ void zwave_kernel(const PassArgs<T> a) {
                 for (int j = 0; j < CH; j++) lv[j] = (T)0.5 * (V0[j] + vx[j] + gv[j] + (qf ? rx[slot][j] : (T)0));
                 if (ok) {
 #pragma unroll
-                    for (int j = 0; j < CH; j++) a.vbuf[off_of(q * CH + j)] = lv[j];
+                    for (int j = 0; j < CH; j++) st_nt(&a.vbuf[off_of(q * CH + j)], lv[j]);
                 }
                 mul_exp_chunk(p0, lv, a.ph.c.x, a.ph.c.y);
 #pragma unroll
@@ -411,7 +437,7 @@ void zwave_kernel(const PassArgs<T> a) {
                 for (int j = 0; j < CH; j++) lv[j] = (T)0.5 * (V0[j] + vx[j] + gv[j] + (qf ? rx[slot][j] : (T)0));
                 if (ok) {
 #pragma unroll
-                    for (int j = 0; j < CH; j++) a.vbuf[off_of(q * CH + j)] = lv[j];
+                    for (int j = 0; j < CH; j++) st_nt(&a.vbuf[off_of(q * CH + j)], lv[j]);
                 }
                 mul_exp_chunk(p0, lv, a.ph.c.x, a.ph.c.y);
 #pragma unroll
@@ -439,7 +465,7 @@ void zwave_kernel(const PassArgs<T> a) {
                 for (int j = 0; j < CH; j++) {
                     const T r0 = density(a.ph, p0[j]);
                     const T qv = (p0[j].x * u[j].y + (-p0[j].y) * u[j].x) / (r0 + (T)1.0e-12);
-                    if (ok) a.vq[off_of(q * CH + j)] = qv;
+                    if (ok) st_nt(&a.vq[off_of(q * CH + j)], qv);
                     lv[j] = vx[j] + gv[j] + qv;
                 }
                 mul_exp_chunk(p0, lv, a.ph.c.x, a.ph.c.y);
@@ -456,7 +482,7 @@ void zwave_kernel(const PassArgs<T> a) {
                 }
                 if (ok) {
 #pragma unroll
-                    for (int j = 0; j < CH; j++) a.psiA[off_of(q * CH + j)] = u[j];
+                    for (int j = 0; j < CH; j++) st_nt(&a.psiA[off_of(q * CH + j)], u[j]);
                 }
                 if (a.flags & F_NEXT_VSTEP1) {
                     dedfdn_chunk(a.ph, u, gv);
