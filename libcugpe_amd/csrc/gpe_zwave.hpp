@@ -57,32 +57,6 @@ struct ZW {
     static constexpr int PENCILS = PW * WAVES;    // pencils per workgroup
 };
 
-// Side arrays (psi0, V1, V_qf) are written by one z pass and read back by
-// the next z pass four axis passes later; the state array in between is
-// read and written by every pass.  Their accesses are non-temporal, so they
-// do not push the state array out of the caches (measured +6% steps/s at
-// 256^3 complex64, bit-identical results).
-template <typename T>
-__device__ __forceinline__ void st_nt(cplx<T>* p, cplx<T> v) {
-    using V2 = T __attribute__((ext_vector_type(2)));
-    V2 w = {v.x, v.y};
-    __builtin_nontemporal_store(w, reinterpret_cast<V2*>(p));
-}
-template <typename T>
-__device__ __forceinline__ cplx<T> ld_nt(const cplx<T>* p) {
-    using V2 = T __attribute__((ext_vector_type(2)));
-    const V2 w = __builtin_nontemporal_load(reinterpret_cast<const V2*>(p));
-    return mk(w.x, w.y);
-}
-template <typename T>
-__device__ __forceinline__ void st_nt(T* p, T v) {
-    __builtin_nontemporal_store(v, p);
-}
-template <typename T>
-__device__ __forceinline__ T ld_nt(const T* p) {
-    return __builtin_nontemporal_load(p);
-}
-
 // load mode of the energy pass: plain load plus the potential-energy sums
 constexpr int LOAD_REDPOT = 3;
 
