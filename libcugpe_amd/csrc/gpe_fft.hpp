@@ -110,6 +110,14 @@ struct Roots16 {
                                      -1.0, -0.92387953251128674, -0.70710678118654752, -0.38268343236508977};
 };
 
+// exact cos / sin of 2 pi j / 12
+struct Roots12 {
+    static constexpr double c[12] = {1.0,  0.86602540378443864676,  0.5, 0.0, -0.5, -0.86602540378443864676,
+                                     -1.0, -0.86602540378443864676, -0.5, 0.0, 0.5,  0.86602540378443864676};
+    static constexpr double s[12] = {0.0,  0.5, 0.86602540378443864676,  1.0,  0.86602540378443864676,  0.5,
+                                     0.0,  -0.5, -0.86602540378443864676, -1.0, -0.86602540378443864676, -0.5};
+};
+
 // exact cos / sin of 2 pi j / 32 for odd j (entry (j - 1) / 2)
 struct Roots32Odd {
     static constexpr double c[16] = {0.98078528040323044913,  0.83146961230254523708,  0.55557023301960222474,
@@ -126,12 +134,36 @@ struct Roots32Odd {
                                      -0.19509032201612826785};
 };
 
-// a * exp(S 2 pi i E / R) for R dividing 32.  complex64 takes a constant
-// as a float hi + lo pair, so that its rounding does not shrink |a| the same
-// way on every application
+// a * exp(S 2 pi i E / R) for R dividing 32 or 12.  complex64 takes a
+// constant as a float hi + lo pair, so that its rounding does not shrink |a|
+// the same way on every application
 template <int R, int E, int S, typename T>
 __device__ __forceinline__ cplx<T> rot(cplx<T> a) {
-    if constexpr (R == 32) {
+    if constexpr (R == 3 || R == 6 || R == 12) {
+        constexpr int j = (E % R) * (12 / R);
+        if constexpr (j == 0) {
+            return a;
+        } else if constexpr (j == 3) {
+            return (S > 0) ? mk(-a.y, a.x) : mk(a.y, -a.x);
+        } else if constexpr (j == 6) {
+            return mk(-a.x, -a.y);
+        } else if constexpr (j == 9) {
+            return (S > 0) ? mk(a.y, -a.x) : mk(-a.y, a.x);
+        } else {
+            constexpr double cv = Roots12::c[j];
+            constexpr double sv = S * Roots12::s[j];
+            if constexpr (sizeof(T) == 4) {
+                constexpr float ch = (float)cv, cl = (float)(cv - (double)ch);
+                constexpr float sh = (float)sv, sl = (float)(sv - (double)sh);
+                return mk(fmaf(a.x, ch, fmaf(-a.y, sh, a.x * cl - a.y * sl)),
+                          fmaf(a.x, sh, fmaf(a.y, ch, a.x * sl + a.y * cl)));
+            } else {
+                const T c = (T)cv;
+                const T s = (T)sv;
+                return mk(a.x * c - a.y * s, a.x * s + a.y * c);
+            }
+        }
+    } else if constexpr (R == 32) {
         constexpr int e = E % 32;
         if constexpr (e % 2 == 0) {
             return rot<16, e / 2, S>(a);
@@ -284,7 +316,9 @@ __device__ __forceinline__ void dft(cplx<T>* v) {
     if constexpr (R == 2) dft2<S>(v);
     else if constexpr (R == 3) dft3<S>(v);
     else if constexpr (R == 4) dft4<S>(v);
+    else if constexpr (R == 6) dft_composite<3, 2, S>(v);
     else if constexpr (R == 8) dft_composite<4, 2, S>(v);
+    else if constexpr (R == 12) dft_composite<4, 3, S>(v);
     else if constexpr (R == 16) dft_composite<4, 4, S>(v);
     else if constexpr (R == 32) dft_composite<8, 4, S>(v);
     else static_assert(R == 2, "unsupported radix");

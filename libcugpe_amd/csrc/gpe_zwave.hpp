@@ -31,8 +31,11 @@ template <typename T>
 constexpr int zw_lanes(int N) {
     switch (N) {
     case 8: return 2;
+    case 12: return 2;
     case 16: return 4;
+    case 24: return 2;
     case 32: return 4;
+    case 48: return 4;
     case 64: return 8;
     case 128: return 8;
     case 256: return 16;
@@ -49,13 +52,22 @@ struct ZW {
     static constexpr int ROW = A + 1;             // padded exchange row
     // exchange rows held at once: all B, or A in two phases when B = 2A > 16
     static constexpr int XR = (B > 16) ? A : B;
-    // exchange elements per pencil, = 16 mod 32 so that consecutive pencils of
-    // a wave start half a bank cycle apart (8-byte elements, 64 banks)
-    static constexpr int GS = XR * ROW + ((16 - (XR * ROW) % 32) + 32) % 32;
+    // exchange elements per pencil: the smallest odd multiple of A that holds
+    // the rows, so the 64/A pencils of a wave start at 32/A different
+    // 8-byte bank slots and a wave's exchange access is at most 2-way
+    static constexpr int GS = ((XR * ROW + A - 1) / A) % 2 ? ((XR * ROW + A - 1) / A) * A
+                                                            : ((XR * ROW + A - 1) / A + 1) * A;
     static constexpr int WAVE = PW * GS;          // exchange elements per wave
     static constexpr int WAVES = kThreads / 64;
     static constexpr int PENCILS = PW * WAVES;    // pencils per workgroup
 };
+
+// the largest power-of-two chunk up to cht that divides the B points of a lane
+constexpr int zw_chunk(int B, int cht) {
+    int c = cht;
+    while (c > 1 && B % c != 0) c /= 2;
+    return c;
+}
 
 // load mode of the energy pass: plain load plus the potential-energy sums
 constexpr int LOAD_REDPOT = 3;
@@ -307,8 +319,9 @@ void zwave_kernel(const PassArgs<T> a) {
     // complex128 holds twice the registers per point, 512-point pencils twice
     // the points per lane: chunks of four
     constexpr int CHT = (sizeof(T) == 4 && N <= 256) ? ZW_CH : 4;
-    constexpr int CH = (B >= CHT) ? CHT : B;
+    constexpr int CH = zw_chunk(B, CHT);
     constexpr int NCH = B / CH;
+    static_assert(NCH * CH == B, "chunks must tile the points of a lane");
     constexpr bool PRE = (LM != LOAD_PLAIN);   // physics before the first transform
     constexpr bool POST = (MM != MID_NONE);    // physics after it
     constexpr bool NEED_AUX = (LM == LOAD_PART2) || MM == MID_VSTEP2 || MM == MID_PART1 ||

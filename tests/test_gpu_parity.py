@@ -41,7 +41,7 @@ def pair(oracle, shape, alpha, beta, dt, npart, interactions, vext, params, prec
 
 
 @pytest.mark.parametrize("shape", [(32, 32, 32), (256, 32, 32), (24, 16, 12), (48, 8, 64), (16, 96, 8), (8, 8, 512),
-                                   (512, 8, 8)])
+                                   (512, 8, 8), (12, 8, 24), (8, 12, 48)])
 @pytest.mark.parametrize("prec", [E.C128, E.C64])
 def test_fft3d_matches_numpy(shape, prec):
     say("fft3d", shape, prec)
@@ -300,7 +300,7 @@ def test_pinned_variants_match_oracle(oracle, variant, mode):
 
 
 @pytest.mark.parametrize("variant", [0, 1, 2, 3])
-@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 8, 64), (16, 96, 8)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 8, 64), (16, 96, 8), (8, 12, 48), (16, 8, 24)])
 def test_launcher_variants_match_oracle(oracle, variant, shape):
     """Every launcher choice of the autotuner (z passes in wave registers or LDS
     tiles, full or half-width strided tiles) follows the oracle's step."""
