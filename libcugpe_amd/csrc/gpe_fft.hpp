@@ -74,8 +74,10 @@ constexpr bool plan_ok(int N, bool c64) {
 // tile width (pencils per workgroup), a power of two
 template <typename T, int N>
 constexpr int tile_w() {
-    // complex64 512: 16 columns (128-byte rows) rather than 8
-    int base = (sizeof(T) == 4) ? (N >= 512 ? 8192 : 4096) : 2048;
+    // complex64 512: 16 columns (128-byte rows) rather than 8; complex128 128
+    // and 256: 32 and 16 columns, so that every thread owns a butterfly of
+    // the radix-16 first stage (256-byte rows)
+    int base = (sizeof(T) == 4) ? (N >= 512 ? 8192 : 4096) : ((N == 128 || N == 256) ? 4096 : 2048);
     int w = base / N;
     if (w < 1) w = 1;
     int p2 = 1;
