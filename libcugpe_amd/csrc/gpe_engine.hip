@@ -547,10 +547,15 @@ int set_psi_t(gpe_engine_s* e, const cuCplx* psi, size_t inst = 0) {
     if (sizeof(T) == sizeof(double)) {
         HIP_TRY(hipMemcpyAsync(dst, psi, e->n * sizeof(cuCplx), hipMemcpyHostToDevice, e->stream));
     } else {
-        std::vector<cplx<T>> tmp(e->n);
-        for (size_t i = 0; i < e->n; i++) tmp[i] = mk((T)psi[i].x, (T)psi[i].y);
-        HIP_TRY(hipMemcpyAsync(dst, tmp.data(), e->n * sizeof(cplx<T>), hipMemcpyHostToDevice, e->stream));
-        HIP_TRY(hipStreamSynchronize(e->stream));
+        // complex128 values staged in the predictor array B (free between
+        // calls), converted on the device: chunks of B's capacity
+        const size_t cap = e->n * e->batch * sizeof(cplx<T>) / sizeof(cuCplx);
+        for (size_t i0 = 0; i0 < e->n; i0 += cap) {
+            const size_t m = std::min(cap, e->n - i0);
+            HIP_TRY(hipMemcpyAsync(e->B, psi + i0, m * sizeof(cuCplx), hipMemcpyHostToDevice, e->stream));
+            launch_c128_to_c64(e->B, dst + i0, m, e->stream);
+            HIP_TRY(hipGetLastError());
+        }
     }
     HIP_TRY(hipStreamSynchronize(e->stream));
     return 0;
@@ -563,13 +568,15 @@ int get_psi_t(gpe_engine_s* e, cuCplx* psi, size_t inst = 0) {
         HIP_TRY(hipMemcpyAsync(psi, src, e->n * sizeof(cuCplx), hipMemcpyDeviceToHost, e->stream));
         HIP_TRY(hipStreamSynchronize(e->stream));
     } else {
-        std::vector<cplx<T>> tmp(e->n);
-        HIP_TRY(hipMemcpyAsync(tmp.data(), src, e->n * sizeof(cplx<T>), hipMemcpyDeviceToHost, e->stream));
-        HIP_TRY(hipStreamSynchronize(e->stream));
-        for (size_t i = 0; i < e->n; i++) {
-            psi[i].x = (double)tmp[i].x;
-            psi[i].y = (double)tmp[i].y;
+        // converted on the device into B, then copied out in chunks of B's capacity
+        const size_t cap = e->n * e->batch * sizeof(cplx<T>) / sizeof(cuCplx);
+        for (size_t i0 = 0; i0 < e->n; i0 += cap) {
+            const size_t m = std::min(cap, e->n - i0);
+            launch_c64_to_c128(src + i0, e->B, m, e->stream);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(psi + i0, e->B, m * sizeof(cuCplx), hipMemcpyDeviceToHost, e->stream));
         }
+        HIP_TRY(hipStreamSynchronize(e->stream));
     }
     return 0;
 }

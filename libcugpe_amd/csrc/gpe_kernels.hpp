@@ -16,6 +16,11 @@ int fft_tile_width(int N);
 template <typename T>
 int fft_lean_width(int N);
 
+// the complex128 host layout of psi to and from a complex64 state, element by
+// element with round-to-nearest (the host buffers of gpe_set_psi / gpe_get_psi)
+void launch_c128_to_c64(const void* src, void* dst, size_t n, hipStream_t s);
+void launch_c64_to_c128(const void* src, void* dst, size_t n, hipStream_t s);
+
 // one FFT axis pass; returns the number of workgroups launched (0 on error)
 template <typename T>
 unsigned launch_fft_pass(int N, const PassArgs<T>& a, hipStream_t s);
