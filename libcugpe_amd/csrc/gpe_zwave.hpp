@@ -11,7 +11,8 @@
 // exchange through the wave's own LDS rows, and A-point DFTs over the lanes.
 // After the exchange lane l owns the columns kb = l + A m (m < B/A), and the
 // output X[l + A m + B ka] lands in register m + (B/A) ka: index l + A b'
-// again.  Every transform maps this layout onto itself, so loads, the
+// again (B is a multiple of A; at 512 points, B = 2A = 32, the exchange runs
+// in two phases of A rows to keep the LDS area of a 256-point pencil).  Every transform maps this layout onto itself, so loads, the
 // pointwise physics before, between and after the transforms, and the stores
 // all work on the same registers.  No workgroup barrier runs between the
 // table set-up and the FP64 reductions at the end; waves of one workgroup
