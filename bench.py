@@ -459,7 +459,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=40)  # about 10 s of the FP64 oracle on 16 threads
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c128", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
