@@ -286,3 +286,98 @@ def test_vortex_variants_closed_forms(oracle):
     assert np.max(np.abs(np.abs(out) - np.abs(psi) * np.sqrt(1.0 / np.sum(np.abs(psi) ** 2)))) < 1e-6
     # get_phase returns arg(psi) and keeps it as the enforced phase
     assert np.allclose(d.get_phase(), np.angle(out), atol=1e-15)
+
+
+def _order_state(shape):
+    ix, iy, iz = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+    nx, ny, nz = shape
+    amp = 1.0 + 0.3 * np.cos(2 * np.pi * ix / nx) * np.sin(2 * np.pi * iy / ny) + 0.2 * np.cos(4 * np.pi * iz / nz)
+    return amp * np.exp(0.4j * np.sin(2 * np.pi * (ix / nx + iz / nz)))
+
+
+@pytest.mark.parametrize(
+    "alpha,beta,qf,inter,order",
+    [
+        (1.0, 0.0, 0.0, CONTACT, 2),  # real time: __gpe_exp_Vstep2_ (orginal/gpe_engine.cu:774-801)
+        (1.0, 0.0, 0.0, UFG, 2),
+        (0.0, 1.0, 0.0, CONTACT, 2),  # imaginary time: part1 + normalise + part2 (803-841, 979-983)
+        (1.0, 0.3, 0.0, CONTACT, 2),  # complex time, same kernels
+        (1.0, 0.0, 1e-9, CONTACT, 2),  # the QF kernels (part2_qf, 843-868) with negligible friction
+        (1.0, 0.0, 0.5, CONTACT, 1),  # V_qf frozen at psi0 in both halves (953, 987): first order
+    ],
+)
+def test_temporal_order_of_the_nonlinear_step(oracle, alpha, beta, qf, inter, order):
+    """Error at time T against a run with dt/1024 must fall by 2^order per
+    halving of dt.  The corrector potential V1 = (V0 + V(phi3, it+1))/2 makes
+    the nonlinear step second order; building V1 from rho(psi0) instead of
+    rho(phi3) (orginal/gpe_engine.cu:792-793, 826-830) leaves it first order,
+    a ratio of 2.  Quantum friction evaluates V_qf from psi0 in both halves
+    of the step, so with a finite coefficient the step is first order."""
+    shape = (16, 12, 8)
+    psi = _order_state(shape)
+    T = 1.6
+
+    def run(dt):
+        e = oracle.OracleEngine(shape, alpha, beta, dt, 50.0, inter, COSINE, 0)
+        e.set_user_params([0.3, 1.0, 0.0, 0.05])
+        if qf:
+            e.set_quantum_friction_coeff(qf)
+        e.set_psi(psi)
+        e.normalize_psi()
+        e.evolve_qf(int(round(T / dt)))
+        out = e.get_psi()[0]
+        e.close()
+        return out
+
+    ref = run(T / 1024)
+    errs = [np.linalg.norm(run(dt) - ref) / np.linalg.norm(ref) for dt in (0.1, 0.05, 0.025)]
+    ratios = [errs[0] / errs[1], errs[1] / errs[2]]
+    want = 2.0 ** order
+    tol = 0.6 if order == 2 else 0.3
+    assert all(abs(r - want) < tol for r in ratios), (errs, ratios)
+
+
+@pytest.mark.parametrize("gam", [1, 2])
+def test_bogoliubov_frequency(oracle, gam):
+    """A small density wave delta cos(k x) on a uniform state of density n0
+    oscillates at the Bogoliubov frequency w^2 = e_k (e_k + 2 g n0) with
+    e_k = k^2/(2 gamma) and g = gamma^2 4 pi a (the step propagates
+    i psi_t = [k^2/(2 gamma) + gamma dEDF/dn(gamma |psi|^2)] psi).  The
+    numerical frequency must converge to it at second order in dt.  With
+    u = psi/<psi> - 1 projected on cos(k x), u = A cos(w t) - i (w/e_k) A sin(w t),
+    so the angle of Re u - i (e_k/w) Im u advances at w."""
+    shape = (32, 4, 4)
+    a, n0, delta = 0.01, 1.5, 1e-4
+    k = 2 * np.pi / shape[0]
+    ek = k * k / (2 * gam)
+    w = np.sqrt(ek * (ek + 2 * 4 * np.pi * a * gam * gam * n0))
+    ix = np.arange(shape[0])[:, None, None] * np.ones(shape)
+    cosk = np.cos(k * ix)
+
+    def omega(dt):
+        e = oracle.OracleEngine(shape, 1.0, 0.0, dt, 1.0, CONTACT, COSINE, gam)
+        e.set_user_params([0.0, 1.0, 0.0, a])
+        e.set_psi(np.sqrt(n0) * (1 + delta * cosk))
+        nsteps = int(round(2 * np.pi / w / dt))
+        chunk = max(1, nsteps // 16)
+        ang, prev, done = 0.0, 0.0, 0
+        amps = []
+        while done < nsteps:
+            s = min(chunk, nsteps - done)
+            e.evolve_qf(s)
+            done += s
+            p = e.get_psi()[0]
+            u = p / p.mean() - 1
+            au = np.sum(u * cosk) / np.sum(cosk * cosk)
+            z = au.real - 1j * (ek / w) * au.imag
+            amps.append(abs(z))
+            d = (np.angle(z) - prev + np.pi) % (2 * np.pi) - np.pi
+            ang += d
+            prev = np.angle(z)
+        e.close()
+        assert np.max(np.abs(np.array(amps) / delta - 1)) < 1e-3  # a neutral mode, no growth or decay
+        return ang / (done * dt)
+
+    e1, e2 = (abs(omega(dt) - w) / w for dt in (0.2, 0.1))
+    assert e2 < 2e-5, e2
+    assert 3.4 < e1 / e2 < 4.6, (e1, e2)
