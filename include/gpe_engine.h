@@ -237,7 +237,9 @@ int gpe_engine_set_graph(gpe_engine_t e, int on);
  * discrete adjoint of the split step, recomputing one segment of states at a
  * time; psi stays psi(T).  Real time without quantum friction, INTERACTIONS
  * 0, -1 or -2 (GPE_ERR_ARG otherwise); V_ext any VEXT (gradient with respect
- * to the per-point values the step uses). */
+ * to the per-point values the step uses).  backward returns GPE_ERR_ARG when
+ * a parameter of the engine (user parameters, potential, alpha/beta, quantum
+ * friction, launcher variant, time origin) changed after the forward run. */
 int gpe_engine_adjoint_forward(gpe_engine_t e, int nt, const cuCplx *psi_target, int ckpt, double *loss);
 int gpe_engine_adjoint_backward(gpe_engine_t e, double *grad_vext);
 
@@ -259,6 +261,10 @@ int gpe_ensemble_create(gpe_ensemble_t *out, int ninst, const int *dims, int int
 int gpe_ensemble_destroy(gpe_ensemble_t s);
 int gpe_ensemble_set_user_params(gpe_ensemble_t s, int size, const double *params);
 int gpe_ensemble_set_quantum_friction_coeff(gpe_ensemble_t s, double qfcoeff);
+/* VEXT 3 ensembles: the potential array of lattice inst (nx*ny*nz doubles of
+ * its shape); every lattice needs one before gpe_ensemble_evolve, which
+ * otherwise returns GPE_ERR_ARG without launching anything */
+int gpe_ensemble_set_external_potential(gpe_ensemble_t s, int inst, const double *v);
 int gpe_ensemble_set_psi(gpe_ensemble_t s, int inst, const cuCplx *psi);
 int gpe_ensemble_get_psi(gpe_ensemble_t s, int inst, double *t, cuCplx *psi);
 int gpe_ensemble_evolve(gpe_ensemble_t s, int nt);

@@ -23,10 +23,15 @@ class OverlapLoss(torch.autograd.Function):
         eng.set_psi(psi0)
         loss = eng.adjoint_forward(nt, psi_target, ckpt)
         ctx.eng = eng
+        ctx.generation = eng.adjoint_generation
         return torch.tensor(loss, dtype=V.dtype, device=V.device)
 
     @staticmethod
     def backward(ctx, grad_out):
+        # the engine holds the checkpoints of its last forward run only
+        if ctx.eng.adjoint_generation != ctx.generation:
+            raise RuntimeError("OverlapLoss.backward: the engine ran another adjoint forward after this loss was "
+                               "computed; use one engine per live loss")
         g = torch.from_numpy(ctx.eng.adjoint_backward()).to(device=grad_out.device, dtype=grad_out.dtype)
         return grad_out * g, None, None, None, None, None
 

@@ -59,6 +59,7 @@ struct gpe_engine_s {
     double vx0 = 0, vy0 = 0;
     int vQ = 1;
     bool vortex_set = false, phase_set = false, varr_set = false;
+    std::vector<char> varr_slots;  // lattices of the batch whose VEXT 3 array has been set
 
     std::vector<double> kx, ky, kz;
     std::vector<std::complex<double>> Kx, Ky, Kz;
@@ -88,6 +89,7 @@ struct gpe_engine_s {
     int adj_nt = 0, adj_C = 0;
     unsigned adj_it0 = 0;
     bool adj_ready = false;
+    unsigned adj_gen = 0;  // gen at the forward run: a parameter change since then voids the backward run
     double adj_fr = 0, adj_fi = 0;  // lambda(T) = (fr + i fi) psi_target
 
     // captured step graphs (open, middle step, last step) for one step plan;
@@ -699,6 +701,14 @@ int create_impl(gpe_engine_s** out, const gpe_config_t* cfg, double alpha, doubl
     return 0;
 }
 
+// Every entry point that launches step or energy kernels calls this first:
+// a VEXT 3 engine reads its potential array in every such kernel, so a
+// missing array is an argument error raised on the host, before any launch.
+int ready_to_launch(const gpe_engine_s* e) {
+    if (e->vext == VEXT_ARRAY && !e->varr_set) return GPE_ERR_ARG;
+    return 0;
+}
+
 template <typename T>
 int detect_t(gpe_engine_s* e, double rho_min, int8_t* w, long* count) {
     if (!e->wind) HIP_TRY(hipMalloc(&e->wind, 3 * e->n));
@@ -755,6 +765,7 @@ int gpe_engine_change_alpha_beta(gpe_engine_t e, double alpha, double beta) {
 
 int gpe_engine_set_time(gpe_engine_t e, double t0) {
     if (!e) return GPE_ERR_NO_ENGINE;
+    e->gen++;
     e->t0 = t0;
     e->it = 0;
     return 0;
@@ -800,24 +811,33 @@ int gpe_engine_set_vortex(gpe_engine_t e, double x0, double y0, int Q) {
     return 0;
 }
 
-int gpe_engine_set_external_potential(gpe_engine_t e, const double* v) {
-    if (!e) return GPE_ERR_NO_ENGINE;
+// The VEXT 3 array of lattice `slot` of a batch (all lattices when slot < 0).
+// The kernels index the array by the global offset, one copy per lattice.
+static int set_varr_slots(gpe_engine_s* e, long slot, const double* v) {
     e->gen++;
-    if (!v) return GPE_ERR_ARG;
     const size_t es = esize(e);
-    // one copy per lattice of a batch: the kernels index the array by the global offset
     if (!e->varr) HIP_TRY(hipMalloc(&e->varr, e->n * e->batch * es));
     HIP_TRY(hipStreamSynchronize(e->stream));
+    std::vector<float> tmp;
+    if (es == 4) tmp.assign(v, v + e->n);
     for (size_t b = 0; b < e->batch; b++) {
-        if (es == 8) {
+        if (slot >= 0 && b != (size_t)slot) continue;
+        if (es == 8)
             HIP_TRY(hipMemcpy((double*)e->varr + b * e->n, v, e->n * sizeof(double), hipMemcpyHostToDevice));
-        } else {
-            std::vector<float> tmp(v, v + e->n);
+        else
             HIP_TRY(hipMemcpy((float*)e->varr + b * e->n, tmp.data(), e->n * sizeof(float), hipMemcpyHostToDevice));
-        }
     }
-    e->varr_set = true;
+    if (e->varr_slots.size() != e->batch) e->varr_slots.assign(e->batch, 0);
+    for (size_t b = 0; b < e->batch; b++)
+        if (slot < 0 || b == (size_t)slot) e->varr_slots[b] = 1;
+    e->varr_set = std::all_of(e->varr_slots.begin(), e->varr_slots.end(), [](char c) { return c != 0; });
     return 0;
+}
+
+int gpe_engine_set_external_potential(gpe_engine_t e, const double* v) {
+    if (!e) return GPE_ERR_NO_ENGINE;
+    if (!v) return GPE_ERR_ARG;
+    return set_varr_slots(e, -1, v);
 }
 
 // orginal/gpe_engine.cu:699-717
@@ -861,7 +881,7 @@ static int evolve_any(gpe_engine_t e, int nt, double* mu, int pin, bool mid_pin)
     if (!e) return GPE_ERR_NO_ENGINE;
     if ((pin == 1 || pin == 2) && !e->vortex_set) return GPE_ERR_VORTEX_NOT_SET;
     if (pin == 3 && !e->phase_set) return GPE_ERR_PHASE_NOT_SET;
-    if (e->vext == VEXT_ARRAY && !e->varr_set) return GPE_ERR_ARG;
+    if (int r = ready_to_launch(e)) return r;
     return e->precision == GPE_PRECISION_C64 ? evolve_t<float>(e, nt, mu, pin, mid_pin)
                                              : evolve_t<double>(e, nt, mu, pin, mid_pin);
 }
@@ -880,7 +900,7 @@ int gpe_engine_evolve_enforced_phase(gpe_engine_t e, int nt, double* mu) { retur
 int gpe_engine_energy(gpe_engine_t e, double* t, double* ekin, double* eint, double* eext) {
     if (!e) return GPE_ERR_NO_ENGINE;
     if (!t || !ekin || !eint || !eext) return GPE_ERR_ARG;
-    if (e->vext == VEXT_ARRAY && !e->varr_set) return GPE_ERR_ARG;
+    if (int r = ready_to_launch(e)) return r;
     return DISPATCH(e, energy_t, e, t, ekin, eint, eext);
 }
 
@@ -1247,6 +1267,7 @@ extern "C" {
 int gpe_engine_evolve_dipolar(gpe_engine_t e, int nt, double* chemical_potential) {
     if (!e) return GPE_ERR_NO_ENGINE;
     if (e->interactions != INT_DIPOLAR || e->batch != 1 || nt < 0) return GPE_ERR_ARG;
+    if (int r = ready_to_launch(e)) return r;
     return DISPATCH(e, evolve_dipolar_t, e, nt, chemical_potential);
 }
 
@@ -1254,6 +1275,7 @@ int gpe_engine_energy_dipolar(gpe_engine_t e, int method, double* t, double* edi
     if (!e) return GPE_ERR_NO_ENGINE;
     if (e->interactions != INT_DIPOLAR || e->batch != 1 || (method != 1 && method != 2) || !t || !edip)
         return GPE_ERR_ARG;
+    if (int r = ready_to_launch(e)) return r;
     return DISPATCH(e, energy_dipolar_t, e, method, t, edip);
 }
 
@@ -1358,6 +1380,7 @@ int adjoint_forward_t(gpe_engine_s* e, int nt, const cuCplx* tgt, int ckpt, doub
     e->adj_fi = -sc * s4[1];
     e->adj_nt = nt;
     e->adj_C = C;
+    e->adj_gen = e->gen;
     e->adj_ready = true;
     return 0;
 }
@@ -1416,6 +1439,7 @@ extern "C" {
 int gpe_engine_adjoint_forward(gpe_engine_t e, int nt, const cuCplx* psi_target, int ckpt, double* loss) {
     if (!e) return GPE_ERR_NO_ENGINE;
     if (nt < 1 || !psi_target || !loss || ckpt < 0 || !adjoint_supported(e)) return GPE_ERR_ARG;
+    if (int r = ready_to_launch(e)) return r;
     HIP_TRY(hipStreamSynchronize(e->stream));
     return DISPATCH(e, adjoint_forward_t, e, nt, psi_target, ckpt, loss);
 }
@@ -1423,6 +1447,9 @@ int gpe_engine_adjoint_forward(gpe_engine_t e, int nt, const cuCplx* psi_target,
 int gpe_engine_adjoint_backward(gpe_engine_t e, double* grad_vext) {
     if (!e) return GPE_ERR_NO_ENGINE;
     if (!grad_vext || !e->adj_ready || !adjoint_supported(e)) return GPE_ERR_ARG;
+    // the backward run recomputes the forward trajectory with the engine's
+    // current parameters: they must be those of the forward run
+    if (e->adj_gen != e->gen) return GPE_ERR_ARG;
     return DISPATCH(e, adjoint_backward_t, e, grad_vext);
 }
 
@@ -1512,8 +1539,10 @@ int ens_build_graphs(gpe_ensemble_s* s) {
 template <typename T>
 int ens_evolve_t(gpe_ensemble_s* s, int nt) {
     if (nt <= 0) return 0;
-    for (auto* g : s->groups)
+    for (auto* g : s->groups) {
         if (g->beta != 0.0) return GPE_ERR_ARG;  // real time only: one norm per lattice is not kept
+        if (int r = ready_to_launch(g)) return r;
+    }
     for (auto* g : s->groups) HIP_TRY(hipStreamSynchronize(g->stream));
     if (s->use_graph) {
         if (!s->graphs_ok) {
@@ -1588,6 +1617,7 @@ extern "C" {
 
 int gpe_engine_autotune(gpe_engine_t e, int steps, int* variant) {
     if (!e) return GPE_ERR_NO_ENGINE;
+    if (int r = ready_to_launch(e)) return r;
     e->gen++;
     HIP_TRY(hipStreamSynchronize(e->stream));
     return DISPATCH(e, autotune_t, e, steps, variant);
@@ -1684,6 +1714,15 @@ int gpe_ensemble_set_quantum_friction_coeff(gpe_ensemble_t s, double qf) {
     return 0;
 }
 
+int gpe_ensemble_set_external_potential(gpe_ensemble_t s, int inst, const double* v) {
+    if (!s) return GPE_ERR_NO_ENGINE;
+    if (inst < 0 || inst >= (int)s->inst_group.size() || !v) return GPE_ERR_ARG;
+    gpe_engine_s* g = s->groups[s->inst_group[inst]];
+    if (g->vext != VEXT_ARRAY) return GPE_ERR_ARG;
+    ens_drop_graphs(s);
+    return set_varr_slots(g, (long)s->inst_slot[inst], v);
+}
+
 int gpe_ensemble_set_psi(gpe_ensemble_t s, int inst, const cuCplx* psi) {
     if (!s) return GPE_ERR_NO_ENGINE;
     if (inst < 0 || inst >= (int)s->inst_group.size() || !psi) return GPE_ERR_ARG;
@@ -1715,6 +1754,8 @@ int gpe_ensemble_set_graph(gpe_ensemble_t s, int on) {
 
 int gpe_ensemble_autotune(gpe_ensemble_t s, int steps) {
     if (!s) return GPE_ERR_NO_ENGINE;
+    for (auto* g : s->groups)
+        if (int r = ready_to_launch(g)) return r;
     ens_drop_graphs(s);
     for (auto* g : s->groups) {
         const int r = gpe_engine_autotune(g, steps, nullptr);

@@ -92,6 +92,7 @@ def lib():
             "gpe_ensemble_set_user_params": [_p, _i, _p],
             "gpe_ensemble_set_quantum_friction_coeff": [_p, _d],
             "gpe_ensemble_set_psi": [_p, _i, _p],
+            "gpe_ensemble_set_external_potential": [_p, _i, _p],
             "gpe_ensemble_get_psi": [_p, _i, _dp, _p],
             "gpe_ensemble_evolve": [_p, _i],
             "gpe_ensemble_set_graph": [_p, _i],
@@ -318,6 +319,7 @@ class Engine:
         loss = ctypes.c_double()
         _check(lib().gpe_engine_adjoint_forward(self._h, int(nt), _ptr(pt), int(ckpt), ctypes.byref(loss)),
                "gpe_engine_adjoint_forward")
+        self.adjoint_generation = getattr(self, "adjoint_generation", 0) + 1
         return loss.value
 
     def adjoint_backward(self):
@@ -375,6 +377,12 @@ class Ensemble:
 
     def set_quantum_friction_coeff(self, qf):
         _check(lib().gpe_ensemble_set_quantum_friction_coeff(self.h, qf), "gpe_ensemble_set_quantum_friction_coeff")
+
+    def set_external_potential(self, inst, v):
+        a = np.ascontiguousarray(np.asarray(v, dtype=np.float64))
+        if a.shape != self.shapes[inst]:
+            raise ValueError("potential shape %s, lattice %d is %s" % (a.shape, inst, self.shapes[inst]))
+        _check(lib().gpe_ensemble_set_external_potential(self.h, inst, _ptr(a)), "gpe_ensemble_set_external_potential")
 
     def set_psi(self, inst, psi):
         a = np.ascontiguousarray(np.asarray(psi, dtype=np.complex128))
