@@ -261,6 +261,11 @@ int gpe_ensemble_create(gpe_ensemble_t *out, int ninst, const int *dims, int int
 int gpe_ensemble_destroy(gpe_ensemble_t s);
 int gpe_ensemble_set_user_params(gpe_ensemble_t s, int size, const double *params);
 int gpe_ensemble_set_quantum_friction_coeff(gpe_ensemble_t s, double qfcoeff);
+/* Parameter sweeps: the user parameters of lattice inst alone, read by the
+ * user hooks as its d_user_param (INTERACTIONS or VEXT 100; GPE_ERR_ARG for
+ * built-in physics, which reads the ensemble's gpe_ensemble_set_user_params).
+ * Copies `size` values and zeroes the rest, like gpe_set_user_params. */
+int gpe_ensemble_set_instance_params(gpe_ensemble_t s, int inst, int size, const double *params);
 /* VEXT 3 ensembles: the potential array of lattice inst (nx*ny*nz doubles of
  * its shape); every lattice needs one before gpe_ensemble_evolve, which
  * otherwise returns GPE_ERR_ARG without launching anything */

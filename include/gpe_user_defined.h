@@ -19,6 +19,10 @@
  *                 2  cosine lattice A (cos(2 pi P ix/nx) + cos(2 pi P iy/ny) + cos(2 pi P iz/nz))
  *                                                                              (V_AMP, V_PERIODS)
  *                 3  caller-supplied array (gpe_set_external_potential)
+ *   INTERACTIONS / VEXT 100 (GPE_USER): the user hook file compiled into the
+ *                 library (include/gpe_user_hooks.h, or the file named by
+ *                 GPE_USER_HOOKS at build time): gpe_EDF / gpe_dEDFdn,
+ *                 gpe_external_potential and gpe_modify_psi with the time index
  *   GPE_FOR       PARTICLES (gamma = 1) or DIMERS (gamma = 2); defaults as in the
  *                 reference: DIMERS for INTERACTIONS 0 and 1, PARTICLES otherwise.
  */
@@ -29,6 +33,7 @@
 
 #define PARTICLES 1
 #define DIMERS 2
+#define GPE_USER 100
 
 #ifndef INTERACTIONS
 #define INTERACTIONS 0

@@ -5,8 +5,13 @@ precision for the other kernels, one for the host engine / C ABI; objects are
 compiled in parallel into libcugpe_amd/_build/ and linked into the shared
 library in the package directory (in-tree, so it travels with the tree).
 
+The user hook file (the device functions of INTERACTIONS / VEXT 100, in the
+reference's gpe_user_defined.h names) is include/gpe_user_hooks.h unless the
+environment variable GPE_USER_HOOKS names another file.
+
     python -m libcugpe_amd.build            # incremental
     python -m libcugpe_amd.build --force    # rebuild everything
+    GPE_USER_HOOKS=my_hooks.h python -m libcugpe_amd.build --force
 """
 import argparse
 import concurrent.futures as cf
@@ -52,8 +57,16 @@ COMMON = [
 ]
 
 
+def user_hooks():
+    """Absolute path of the user hook file compiled into the library."""
+    return os.path.abspath(os.environ.get("GPE_USER_HOOKS") or os.path.join(INCLUDE, "gpe_user_hooks.h"))
+
+
 def jobs():
-    hdrs = glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(INCLUDE, "*.h"))
+    hooks = user_hooks()
+    if not os.path.exists(hooks):
+        raise RuntimeError("user hook file not found: %s" % hooks)
+    hdrs = glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(INCLUDE, "*.h")) + [hooks]
     out = []
     inst = os.path.join(CSRC, "gpe_fft_inst.hip")
     for real in ("float", "double"):
@@ -75,7 +88,7 @@ def stale(obj, deps):
 
 def compile_one(cc, obj, srcs, defs):
     tmp = obj + ".tmp.o"
-    cmd = [cc] + COMMON + defs + ["-c"] + srcs + ["-o", tmp]
+    cmd = [cc] + COMMON + ['-DGPE_USER_HOOKS_FILE="%s"' % user_hooks()] + defs + ["-c"] + srcs + ["-o", tmp]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (obj, " ".join(cmd), r.stdout))

@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kThreads) void adj_vfill(Phys<T> ph, double* __rest
     const unsigned nyz = (unsigned)(ph.ny * ph.nz);
     const unsigned ix = (unsigned)(i / nyz), r = (unsigned)(i % nyz);
     const unsigned iy = r / (unsigned)ph.nz, iz = r % (unsigned)ph.nz;
-    out[i] = (double)vext(ph, ix, iy, iz, (unsigned)i);
+    out[i] = (double)vext(ph, ix, iy, iz, (unsigned)i, ph.it);
 }
 
 // per workgroup: Re and Im of sum conj(t) psi, sum |a|^2, sum |t|^2

@@ -14,6 +14,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// the user hook file compiled into the library (build.py passes the file named
+// by GPE_USER_HOOKS; include/ is on the include path)
+#ifndef GPE_USER_HOOKS_FILE
+#define GPE_USER_HOOKS_FILE "gpe_user_hooks.h"
+#endif
+
 namespace gpe {
 
 template <typename R>
@@ -76,24 +82,81 @@ constexpr int kMaxUserParams = 32;
 
 // Physics codes: INTERACTIONS (dev/gpe_user_defined.h:90,169,276,356) and
 // VEXT (dev/gpe_user_defined.h:476,520) of the reference, plus VEXT 2 (cosine
-// lattice) and VEXT 3 (caller-supplied array) of this engine.
-enum { INT_UFG = 0, INT_ETF = 1, INT_CONTACT = -1, INT_DIPOLAR = -2 };
-enum { VEXT_HARMONIC = 0, VEXT_WOODS_SAXON = 1, VEXT_COSINE = 2, VEXT_ARRAY = 3 };
+// lattice) and VEXT 3 (caller-supplied array) of this engine, and 100 for the
+// user hook file compiled into the library (gpe_user_hooks.h by default).
+enum { INT_UFG = 0, INT_ETF = 1, INT_CONTACT = -1, INT_DIPOLAR = -2, INT_USER = 100 };
+enum { VEXT_HARMONIC = 0, VEXT_WOODS_SAXON = 1, VEXT_COSINE = 2, VEXT_ARRAY = 3, VEXT_USER = 100 };
 
 // Per-launch physics / step constants (kernel argument, uniform per launch).
 template <typename R>
 struct Phys {
     int nx, ny, nz;
     int interactions, vext;
-    unsigned it;             // time index of the current step
+    unsigned it;             // time index of the launch (host value)
+    const unsigned* itp;     // time index in device memory (step passes with user hooks), or null
+    unsigned it_off;         // added to the time index (the next step's Vstep1 in its own pass)
+    int modify;              // apply the user gpe_modify_psi
     R gamma;                 // 1 particles, 2 dimers
     cplx<R> c;               // step coefficient gamma*dt/2/(-beta + i alpha)
     R up[kMaxUserParams];    // user parameters (d_user_param)
+    const double* uparam;    // user parameters in FP64 per lattice [batch][kMaxUserParams] (user hooks)
+    unsigned n1;             // points per lattice
+    double dt, t0, npart;    // d_dt, d_t0, d_npart of the user hooks
     const R* varr;           // VEXT_ARRAY potential
     const R* vtab;           // separable potentials: per-axis terms [nx | ny | nz]
     R vx0, vy0;              // vortex line position
     R vQ;                    // vortex charge
 };
+
+// time index of a launch: the device counter of a captured step, else the host value
+template <typename R>
+__device__ __forceinline__ unsigned time_index(const Phys<R>& p) {
+    return (p.itp ? *p.itp : p.it) + p.it_off;
+}
+
+// The user hook file, compiled as member functions of a context that supplies
+// the names the reference's hooks use (d_user_param, d_nx .. d_npart, NX .. NZ,
+// cuDoubleComplex) for one lattice.
+struct UserHooks {
+    typedef unsigned int uint;
+    typedef cplx<double> cuDoubleComplex;
+    typedef cplx<double> Complex;
+    const double* d_user_param;
+    uint d_nx, d_ny, d_nz;
+    double d_dt, d_t0, d_npart;
+    static __device__ __forceinline__ cuDoubleComplex make_cuDoubleComplex(double r, double i) { return mk(r, i); }
+    static __device__ __forceinline__ double cuCreal(cuDoubleComplex a) { return a.x; }
+    static __device__ __forceinline__ double cuCimag(cuDoubleComplex a) { return a.y; }
+    static __device__ __forceinline__ cuDoubleComplex cuCadd(cuDoubleComplex a, cuDoubleComplex b) { return cadd(a, b); }
+    static __device__ __forceinline__ cuDoubleComplex cuCsub(cuDoubleComplex a, cuDoubleComplex b) { return csub(a, b); }
+    static __device__ __forceinline__ cuDoubleComplex cuCmul(cuDoubleComplex a, cuDoubleComplex b) { return cmul(a, b); }
+    static __device__ __forceinline__ cuDoubleComplex cuConj(cuDoubleComplex a) { return mk(a.x, -a.y); }
+    static __device__ __forceinline__ double cuCabs(cuDoubleComplex a) { return hypot(a.x, a.y); }
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+#define NX d_nx
+#define NY d_ny
+#define NZ d_nz
+#include GPE_USER_HOOKS_FILE
+#undef NX
+#undef NY
+#undef NZ
+};
+
+// the hook context of the lattice holding global offset `off`
+template <typename R>
+__device__ __forceinline__ UserHooks user_hooks(const Phys<R>& p, unsigned off) {
+    UserHooks h;
+    h.d_user_param = p.uparam + (size_t)(off / p.n1) * kMaxUserParams;
+    h.d_nx = (unsigned)p.nx;
+    h.d_ny = (unsigned)p.ny;
+    h.d_nz = (unsigned)p.nz;
+    h.d_dt = p.dt;
+    h.d_t0 = p.t0;
+    h.d_npart = p.npart;
+    return h;
+}
 
 // gpe_density: orginal/gpe_engine.cu:384-387
 template <typename R>
@@ -101,9 +164,17 @@ __device__ __forceinline__ R density(const Phys<R>& p, cplx<R> v) {
     return p.gamma * (v.x * v.x + v.y * v.y);
 }
 
-// gpe_external_potential (hooks listed above); ix, iy, iz lattice indices
-template <typename R>
-__device__ __forceinline__ R vext(const Phys<R>& p, unsigned ix, unsigned iy, unsigned iz, unsigned off) {
+// The physics functions below take the user hook file only in kernels
+// instantiated with UH = true (the passes of engines with INTERACTIONS or
+// VEXT 100); every other kernel is compiled without the hook code.
+
+// gpe_external_potential (hooks listed above) at time index it; ix, iy, iz
+// lattice indices, off the global offset of the point
+template <bool UH = false, typename R>
+__device__ __forceinline__ R vext(const Phys<R>& p, unsigned ix, unsigned iy, unsigned iz, unsigned off, unsigned it) {
+    if constexpr (UH) {
+        if (p.vext == VEXT_USER) return (R)user_hooks(p, off).gpe_external_potential(ix, iy, iz, it);
+    }
     switch (p.vext) {
     case VEXT_HARMONIC:
         // 0.5 x^2 wx^2 + 0.5 y^2 wy^2 + 0.5 z^2 wz^2, each term tabulated per axis
@@ -125,10 +196,14 @@ __device__ __forceinline__ R vext(const Phys<R>& p, unsigned ix, unsigned iy, un
     }
 }
 
-// gpe_dEDFdn
-template <typename R>
-__device__ __forceinline__ R dedfdn(const Phys<R>& p, R rho) {
+// gpe_dEDFdn at time index it (off: the point's global offset, for the user
+// hooks' per-lattice parameters)
+template <bool UH = false, typename R>
+__device__ __forceinline__ R dedfdn(const Phys<R>& p, R rho, unsigned it, unsigned off) {
     const R PI = (R)3.14159265358979323846;
+    if constexpr (UH) {
+        if (p.interactions == INT_USER) return (R)user_hooks(p, off).gpe_dEDFdn((double)rho, it);
+    }
     switch (p.interactions) {
     case INT_UFG:
         return (R)0.37 * g_pow((R)3.0 * PI * PI * rho, (R)(2.0 / 3.0)) / (R)2.0;
@@ -153,10 +228,15 @@ __device__ __forceinline__ R dedfdn(const Phys<R>& p, R rho) {
     }
 }
 
-// gpe_EDF (energy density); evaluated in FP64 by the energy reductions
-__device__ __forceinline__ double edf(int interactions, const double* up, double rho) {
+// gpe_EDF (energy density) at time index it; evaluated in FP64 by the energy
+// reductions (up: the user parameters in FP64)
+template <bool UH = false, typename R>
+__device__ __forceinline__ double edf(const Phys<R>& p, const double* up, double rho, unsigned it, unsigned off) {
     const double PI = 3.14159265358979323846;
-    switch (interactions) {
+    if constexpr (UH) {
+        if (p.interactions == INT_USER) return user_hooks(p, off).gpe_EDF(rho, it);
+    }
+    switch (p.interactions) {
     case INT_UFG:
         return 0.37 * 0.6 * rho * pow(3.0 * PI * PI * rho, 2.0 / 3.0) / 2.;
     case INT_ETF: {
@@ -175,6 +255,19 @@ __device__ __forceinline__ double edf(int interactions, const double* up, double
     }
 }
 
+// gpe_modify_psi of the user hooks at time index it; the identity otherwise
+template <bool UH = false, typename R>
+__device__ __forceinline__ cplx<R> modify_psi(const Phys<R>& p, cplx<R> v, unsigned ix, unsigned iy, unsigned iz,
+                                              unsigned off, unsigned it) {
+    if constexpr (UH) {
+        if (p.modify) {
+            const cplx<double> r = user_hooks(p, off).gpe_modify_psi(ix, iy, iz, it, mk((double)v.x, (double)v.y));
+            return mk((R)r.x, (R)r.y);
+        }
+    }
+    return v;
+}
+
 // Vortex-line imprint (dev/gpe_engine.cu:485-511): phase Q*atan2(x, y) with
 // x = ix - nx/2 - x0, y = iy - ny/2 - y0, modulus kept.
 template <typename R>
@@ -186,14 +279,6 @@ __device__ __forceinline__ cplx<R> imprint(const Phys<R>& p, cplx<R> v, unsigned
     R s, c;
     g_sincos(ph, &s, &c);
     return mk(a * c, a * s);
-}
-
-// V(psi) = Vext + dEDFdn(rho(psi)) [+ V_qf]   (orginal/gpe_engine.cu:739, 764)
-template <typename R>
-__device__ __forceinline__ R potential(const Phys<R>& p, cplx<R> v, unsigned ix, unsigned iy, unsigned iz,
-                                       unsigned off, unsigned it) {
-    (void)it;
-    return vext(p, ix, iy, iz, off) + dedfdn(p, density(p, v));
 }
 
 }  // namespace gpe

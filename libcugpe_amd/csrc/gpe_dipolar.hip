@@ -28,8 +28,9 @@ __global__ __launch_bounds__(kThreads) void dipolar_vstep_kernel(cplx<T>* __rest
     const unsigned nyz = (unsigned)(ph.ny * ph.nz);
     const unsigned ix = (unsigned)(i / nyz), r = (unsigned)(i % nyz);
     const unsigned iy = r / (unsigned)ph.nz, iz = r % (unsigned)ph.nz;
-    const cplx<T> l = A[i];
-    const T lv = vext(ph, ix, iy, iz, (unsigned)i) + dedfdn(ph, density(ph, l)) + B[i].x;
+    const cplx<T> l = modify_psi<true>(ph, A[i], ix, iy, iz, (unsigned)i, ph.it);
+    const T lv = vext<true>(ph, ix, iy, iz, (unsigned)i, ph.it) + dedfdn<true>(ph, density(ph, l), ph.it, (unsigned)i) +
+                 B[i].x;
     A[i] = mul_exp(l, ph.c.x * lv, ph.c.y * lv);
 }
 

@@ -74,6 +74,9 @@ struct gpe_engine_s {
     void *Khl = nullptr;                             // complex64: K_x and K_y K_z as hi + lo float pairs
     void *vtab = nullptr;                            // per-axis terms of separable potentials
     double *k64 = nullptr;
+    double *updev = nullptr;     // user parameters in FP64 per lattice [batch][kMaxUserParams] (user hooks)
+    unsigned *itdev = nullptr;   // time index of the running step (user hooks)
+    bool uses_hooks = false;     // INTERACTIONS or VEXT 100: the user hook file, time dependent
     double *partials = nullptr;
     size_t partials_len = 0;
     double *scal = nullptr;  // device scalars
@@ -288,6 +291,14 @@ PassArgs<T> base_args(gpe_engine_s* e) {
     p.gamma = (T)e->gamma;
     p.c = mk((T)e->c.real(), (T)e->c.imag());
     for (int i = 0; i < kMaxUserParams; i++) p.up[i] = (T)e->up[i];
+    p.itp = nullptr;
+    p.it_off = 0;
+    p.modify = e->uses_hooks ? 1 : 0;
+    p.uparam = e->updev;
+    p.n1 = (unsigned)e->n;
+    p.dt = e->dt;
+    p.t0 = e->t0;
+    p.npart = e->npart;
     p.varr = (const T*)e->varr;
     p.vtab = (const T*)e->vtab;
     p.vx0 = (T)e->vx0;
@@ -374,6 +385,9 @@ PassArgs<T> step_args(gpe_engine_s* e, const StepPlan& sp) {
     a.kfac = (T)e->qf;
     a.pin = sp.pin;
     a.phase = e->phase;
+    // user hooks read the step's time index from device memory, so that a
+    // captured step graph stays valid for every step it is replayed for
+    if (e->uses_hooks) a.ph.itp = e->itdev;
     return a;
 }
 
@@ -432,14 +446,17 @@ void issue_step(gpe_engine_s* e, const StepPlan& sp, bool last) {
         const int nb = run_pass<T>(e, a, {2, A, nullptr, I, 0, LOAD_PLAIN, MID_VSTEP3, STORE_NONE, F_RED_NORM, 1});
         finalize(e, nb, 1, FIN_NORM, 0, 1.0);
         if (!last) {
+            PassArgs<T> an = a;
+            an.ph.it_off = 1;  // the next step's Vstep1
             if (!qf)
-                run_pass<T>(e, a, {2, nullptr, B, F, 0, LOAD_PSI0, MID_NONE, STORE_PLAIN, F_SCALE | F_VSTEP1 | imp, 0});
+                run_pass<T>(e, an, {2, nullptr, B, F, 0, LOAD_PSI0, MID_NONE, STORE_PLAIN, F_SCALE | F_VSTEP1 | imp, 0});
             else
-                run_pass<T>(e, a, {2, nullptr, C, F, 0, LOAD_PSI0, MID_NONE, STORE_PLAIN, F_SCALE | imp, 0});
+                run_pass<T>(e, an, {2, nullptr, C, F, 0, LOAD_PSI0, MID_NONE, STORE_PLAIN, F_SCALE | imp, 0});
         } else {
             launch_scale<T>((cplx<T>*)A, (unsigned)e->n, e->scal, e->stream);
         }
     }
+    if (e->uses_hooks) launch_tick(e->itdev, e->stream);
 }
 
 template <typename T>
@@ -452,10 +469,12 @@ int evolve_t(gpe_engine_s* e, int nt, double* mu, int pin, bool mid_pin) {
     StepPlan sp;
     sp.pin = pin;
     sp.mid_pin = mid_pin;
-    // With graphs on, runs of 3+ steps replay captured graphs: the passes' arguments are the
-    // same every step (the time index among them is read by no built-in
-    // potential), so one capture of the open pass, a middle step and the last
-    // step serves every run until a parameter changes.  Not while profiling.
+    if (e->uses_hooks) launch_set_counter(e->itdev, e->it, e->stream);
+    // With graphs on, runs of 3+ steps replay captured graphs: the passes'
+    // arguments are the same every step (the user hooks' time index is read
+    // from device memory, advanced by a kernel at the end of every step), so
+    // one capture of the open pass, a middle step and the last step serves
+    // every run until a parameter changes.  Not while profiling.
     if (e->use_graph && nt >= 3 && e->prof_class < 0 && e->batch == 1) {
         if (e->graph_gen != e->gen || e->graph_pin != pin || e->graph_mid_pin != mid_pin) {
             drop_graphs(e);
@@ -587,7 +606,7 @@ void free_all(gpe_engine_s* e) {
     drop_graphs(e);
     void* ptrs[] = {e->A, e->B, e->C, e->Vb, e->Vq, e->varr, e->tw[0], e->tw[1], e->tw[2], e->twlo[0], e->twlo[1],
                     e->twlo[2], e->Ktab, e->ktab, e->Khl, e->vtab,
-                    e->k64, e->partials, e->scal, e->rout, e->phase, e->wind, e->adj_ck, e->adj_seg,
+                    e->k64, e->updev, e->itdev, e->partials, e->scal, e->rout, e->phase, e->wind, e->adj_ck, e->adj_seg,
                     e->adj_lam, e->adj_phi, e->adj_lam0, e->adj_tgt, e->adj_fin, e->adj_vb, e->adj_grad,
                     e->adj_vx};
     for (void* p : ptrs)
@@ -629,8 +648,8 @@ int create_impl(gpe_engine_s** out, const gpe_config_t* cfg, double alpha, doubl
     if (batch < 1 || n * batch >= (size_t)1 << 31) return GPE_ERR_LATTICE;
     if (cfg->precision != GPE_PRECISION_C64 && cfg->precision != GPE_PRECISION_C128) return GPE_ERR_ARG;
     if (cfg->gpe_for != 0 && cfg->gpe_for != PARTICLES && cfg->gpe_for != DIMERS) return GPE_ERR_GAMMA;
-    if (cfg->vext < 0 || cfg->vext > 3) return GPE_ERR_ARG;
-    if (cfg->interactions < -2 || cfg->interactions > 1) return GPE_ERR_ARG;
+    if ((cfg->vext < 0 || cfg->vext > 3) && cfg->vext != VEXT_USER) return GPE_ERR_ARG;
+    if ((cfg->interactions < -2 || cfg->interactions > 1) && cfg->interactions != INT_USER) return GPE_ERR_ARG;
 
     gpe_engine_s* e = new gpe_engine_s();
     e->nx = cfg->nx;
@@ -641,6 +660,7 @@ int create_impl(gpe_engine_s** out, const gpe_config_t* cfg, double alpha, doubl
     e->precision = cfg->precision;
     e->interactions = cfg->interactions;
     e->vext = cfg->vext;
+    e->uses_hooks = (cfg->interactions == INT_USER || cfg->vext == VEXT_USER);
     e->gpe_for = cfg->gpe_for;
     if (cfg->gpe_for == PARTICLES) e->gamma = 1.0;
     else if (cfg->gpe_for == DIMERS) e->gamma = 2.0;
@@ -690,14 +710,29 @@ int create_impl(gpe_engine_s** out, const gpe_config_t* cfg, double alpha, doubl
     e->partials_len = 3 * maxb;
     ALLOC(e->partials, e->partials_len * sizeof(double));
     ALLOC(e->scal, 16 * sizeof(double));
+    ALLOC(e->updev, batch * kMaxUserParams * sizeof(double));
+    ALLOC(e->itdev, 64);
 #undef ALLOC
     (void)hipMemset(e->A, 0, nb * 2 * es);
     (void)hipMemset(e->scal, 0, 16 * sizeof(double));
+    (void)hipMemset(e->updev, 0, batch * kMaxUserParams * sizeof(double));
+    (void)hipMemset(e->itdev, 0, 64);
     int r = (e->precision == GPE_PRECISION_C64) ? upload_static_tables<float>(e) : upload_static_tables<double>(e);
     if (!r) r = upload_potential_any(e);
     if (r) return fail(r);
     e->variant = default_variant(e);
     *out = e;
+    return 0;
+}
+
+// The FP64 parameter block the user hooks read as d_user_param, for lattice
+// `slot` of the batch (every lattice when slot < 0).
+int upload_user_params(gpe_engine_s* e, long slot, const double* up) {
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (size_t b = 0; b < e->batch; b++)
+        if (slot < 0 || b == (size_t)slot)
+            HIP_TRY(hipMemcpy(e->updev + b * kMaxUserParams, up, kMaxUserParams * sizeof(double),
+                              hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -777,6 +812,8 @@ int gpe_engine_set_user_params(gpe_engine_t e, int size, const double* params) {
     if (size > MAX_USER_PARAMS || size > kMaxUserParams) return GPE_ERR_TOO_MANY_PARAMS;
     if (size < 0 || (size > 0 && !params)) return GPE_ERR_ARG;
     for (int i = 0; i < kMaxUserParams; i++) e->up[i] = (i < size) ? params[i] : 0.0;
+    int r = upload_user_params(e, -1, e->up);
+    if (r) return r;
     return upload_potential_any(e);
 }
 
@@ -1180,7 +1217,7 @@ int gpe_host_free(void* ptr) {
 void gpe_print_potential_type(void) {
     static const char* names[] = {"HARMONIC POTENTIAL", "WOODS-SAXON TUBE", "COSINE LATTICE", "USER ARRAY"};
     int v = g_eng ? g_eng->vext : g_cfg.cfg.vext;
-    printf("%s\n", (v >= 0 && v <= 3) ? names[v] : "UNKNOWN");
+    printf("%s\n", (v >= 0 && v <= 3) ? names[v] : (v == VEXT_USER ? "USER HOOK POTENTIAL" : "UNKNOWN"));
 }
 
 }  // extern "C"
@@ -1428,7 +1465,7 @@ int adjoint_backward_t(gpe_engine_s* e, double* grad) {
 }
 
 bool adjoint_supported(const gpe_engine_s* e) {
-    return e->beta == 0.0 && e->qf == 0.0 && e->batch == 1 &&
+    return e->beta == 0.0 && e->qf == 0.0 && e->batch == 1 && e->vext != VEXT_USER &&
            (e->interactions == INT_UFG || e->interactions == INT_CONTACT || e->interactions == INT_DIPOLAR);
 }
 
@@ -1544,6 +1581,8 @@ int ens_evolve_t(gpe_ensemble_s* s, int nt) {
         if (int r = ready_to_launch(g)) return r;
     }
     for (auto* g : s->groups) HIP_TRY(hipStreamSynchronize(g->stream));
+    for (auto* g : s->groups)
+        if (g->uses_hooks) launch_set_counter(g->itdev, g->it, s->main);
     if (s->use_graph) {
         if (!s->graphs_ok) {
             int r = ens_build_graphs<T>(s);
@@ -1712,6 +1751,22 @@ int gpe_ensemble_set_quantum_friction_coeff(gpe_ensemble_t s, double qf) {
         if (r) return r;
     }
     return 0;
+}
+
+int gpe_ensemble_set_instance_params(gpe_ensemble_t s, int inst, int size, const double* params) {
+    if (!s) return GPE_ERR_NO_ENGINE;
+    if (inst < 0 || inst >= (int)s->inst_group.size()) return GPE_ERR_ARG;
+    if (size > MAX_USER_PARAMS || size > kMaxUserParams) return GPE_ERR_TOO_MANY_PARAMS;
+    if (size < 0 || (size > 0 && !params)) return GPE_ERR_ARG;
+    gpe_engine_s* g = s->groups[s->inst_group[inst]];
+    // the built-in physics reads one parameter set per group; the user hooks
+    // read d_user_param per lattice
+    if (!g->uses_hooks) return GPE_ERR_ARG;
+    double up[kMaxUserParams];
+    for (int i = 0; i < kMaxUserParams; i++) up[i] = (i < size) ? params[i] : 0.0;
+    ens_drop_graphs(s);
+    g->gen++;
+    return upload_user_params(g, (long)s->inst_slot[inst], up);
 }
 
 int gpe_ensemble_set_external_potential(gpe_ensemble_t s, int inst, const double* v) {

@@ -643,27 +643,37 @@ __device__ __forceinline__ cplx<T> mid_op_lean(const PassArgs<T>& a, const Coord
 template <typename T>
 struct ElemIn {
     Coord c;
-    T vx;          // V_ext at the point (time independent hooks: the same at it and it+1)
+    unsigned it;   // time index of the step (time_index of the pass)
+    T vx;          // V_ext at the point at it
+    T vx1;         // V_ext at it + 1 (differs from vx only for a time-dependent user potential)
     cplx<T> aux;   // psi0
     T raux;        // V_qf (F_QF) or V1 (MID_VSTEP3)
 };
 
-// V(psi) = Vext + dEDFdn(rho(psi)) [+ V_qf]   (orginal/gpe_engine.cu:739, 764)
-template <typename T>
-__device__ __forceinline__ T pot2(const PassArgs<T>& a, T vx, cplx<T> v, bool qf, T q) {
-    T lv = vx + dedfdn(a.ph, density(a.ph, v));
+// V(psi) = Vext + dEDFdn(rho(psi)) [+ V_qf] at time index it  (orginal/gpe_engine.cu:739, 764)
+template <bool UH, typename T>
+__device__ __forceinline__ T pot2(const PassArgs<T>& a, T vx, cplx<T> v, bool qf, T q, unsigned it, unsigned off) {
+    T lv = vx + dedfdn<UH>(a.ph, density(a.ph, v), it, off);
     if (qf) lv = lv + q;
     return lv;
 }
 
-template <int LM, typename T>
+// gpe_modify_psi at the element (identity unless the engine runs user hooks)
+template <bool UH, typename T>
+__device__ __forceinline__ cplx<T> modify_at(const PassArgs<T>& a, const ElemIn<T>& e, cplx<T> v, unsigned it) {
+    return modify_psi<UH>(a.ph, v, e.c.ix, e.c.iy, e.c.iz, e.c.off, it);
+}
+
+// UH: compiled with the user hook file (passes of engines with INTERACTIONS or VEXT 100)
+template <int LM, bool UH = false, typename T>
 __device__ __forceinline__ cplx<T> load_op_m(const PassArgs<T>& a, const ElemIn<T>& e, cplx<T> v, double* acc) {
     const bool qf = (a.flags & F_QF) != 0;
+    const unsigned off = e.c.off;
     if constexpr (LM == LOAD_PLAIN) {
         if (a.flags & F_RED_POT) {
             const double rho = dens64((double)a.ph.gamma, (double)v.x, (double)v.y);
             acc[0] += rho * (double)e.vx;
-            acc[1] += edf(a.ph.interactions, a.up64, rho);
+            acc[1] += edf<UH>(a.ph, a.up64, rho, e.it, off);
         }
         return v;
     } else if constexpr (LM == LOAD_PSI0) {
@@ -671,9 +681,10 @@ __device__ __forceinline__ cplx<T> load_op_m(const PassArgs<T>& a, const ElemIn<
         bool wb = false;
         if (a.flags & F_SCALE) { v = cscale(v, (T)a.scal[1]); wb = true; }
         if (a.flags & F_IMPRINT) { v = apply_pin(a, v, e.c); wb = true; }
-        if (wb) a.psiA[e.c.off] = v;
+        if (wb) a.psiA[off] = v;
         if (a.flags & F_VSTEP1) {
-            const T lv = pot2(a, e.vx, v, qf, e.raux);
+            v = modify_at<UH>(a, e, v, e.it);
+            const T lv = pot2<UH>(a, e.vx, v, qf, e.raux, e.it, off);
             v = mul_exp(v, a.ph.c.x * lv, a.ph.c.y * lv);
         }
         return v;
@@ -682,39 +693,42 @@ __device__ __forceinline__ cplx<T> load_op_m(const PassArgs<T>& a, const ElemIn<
         cplx<T> phi3 = v;
         if (a.flags & F_SCALE) phi3 = cscale(phi3, (T)a.scal[1]);
         cplx<T> psi0 = e.aux;
-        const T V0 = pot2(a, e.vx, psi0, qf, e.raux);
+        const T V0 = pot2<UH>(a, e.vx, modify_at<UH>(a, e, psi0, e.it), qf, e.raux, e.it, off);
         T lv;
-        if (qf) lv = (T)0.5 * (V0 + e.vx + dedfdn(a.ph, density(a.ph, phi3)) + e.raux);
-        else lv = (T)0.5 * (V0 + e.vx + dedfdn(a.ph, density(a.ph, phi3)));
-        a.vbuf[e.c.off] = lv;
+        const T d1 = dedfdn<UH>(a.ph, density(a.ph, phi3), e.it + 1, off);
+        if (qf) lv = (T)0.5 * (V0 + e.vx1 + d1 + e.raux);
+        else lv = (T)0.5 * (V0 + e.vx1 + d1);
+        a.vbuf[off] = lv;
         if (a.flags & F_IMPRINT_MID) psi0 = apply_pin(a, psi0, e.c);
-        return mul_exp(psi0, a.ph.c.x * lv, a.ph.c.y * lv);
+        return mul_exp(modify_at<UH>(a, e, psi0, e.it), a.ph.c.x * lv, a.ph.c.y * lv);
     }
 }
 
-template <typename T>
+template <bool UH, typename T>
 __device__ __forceinline__ cplx<T> load_op2(const PassArgs<T>& a, const ElemIn<T>& e, cplx<T> v, double* acc) {
-    if (a.load_mode == LOAD_PLAIN) return load_op_m<LOAD_PLAIN>(a, e, v, acc);
-    if (a.load_mode == LOAD_PSI0) return load_op_m<LOAD_PSI0>(a, e, v, acc);
-    return load_op_m<LOAD_PART2>(a, e, v, acc);
+    if (a.load_mode == LOAD_PLAIN) return load_op_m<LOAD_PLAIN, UH>(a, e, v, acc);
+    if (a.load_mode == LOAD_PSI0) return load_op_m<LOAD_PSI0, UH>(a, e, v, acc);
+    return load_op_m<LOAD_PART2, UH>(a, e, v, acc);
 }
 
-template <int MM, typename T>
+template <int MM, bool UH = false, typename T>
 __device__ __forceinline__ cplx<T> mid_op_m(const PassArgs<T>& a, const ElemIn<T>& e, cplx<T> v, double* acc) {
     const bool qf = (a.flags & F_QF) != 0;
+    const unsigned off = e.c.off;
     if constexpr (MM == MID_VSTEP2) {
         // orginal/gpe_engine.cu:774-801 (and the _part1/_part2[_qf] pair for qf)
-        const cplx<T> psi0 = e.aux;
-        const T V0 = pot2(a, e.vx, psi0, qf, e.raux);
+        const cplx<T> psi0 = modify_at<UH>(a, e, e.aux, e.it);
+        const T V0 = pot2<UH>(a, e.vx, psi0, qf, e.raux, e.it, off);
         const cplx<T> phi3 = mul_exp(v, a.ph.c.x * V0, a.ph.c.y * V0);
         T lv;
-        if (qf) lv = (T)0.5 * (V0 + e.vx + dedfdn(a.ph, density(a.ph, phi3)) + e.raux);
-        else lv = (T)0.5 * (V0 + e.vx + dedfdn(a.ph, density(a.ph, phi3)));
-        a.vbuf[e.c.off] = lv;
+        const T d1 = dedfdn<UH>(a.ph, density(a.ph, phi3), e.it + 1, off);
+        if (qf) lv = (T)0.5 * (V0 + e.vx1 + d1 + e.raux);
+        else lv = (T)0.5 * (V0 + e.vx1 + d1);
+        a.vbuf[off] = lv;
         return mul_exp(psi0, a.ph.c.x * lv, a.ph.c.y * lv);
     } else if constexpr (MM == MID_PART1) {
         // orginal/gpe_engine.cu:803-815 with the density sum of gpe_normalize fused
-        const T V0 = pot2(a, e.vx, e.aux, qf, e.raux);
+        const T V0 = pot2<UH>(a, e.vx, modify_at<UH>(a, e, e.aux, e.it), qf, e.raux, e.it, off);
         const cplx<T> phi3 = mul_exp(v, a.ph.c.x * V0, a.ph.c.y * V0);
         acc[0] += dens64((double)a.ph.gamma, (double)phi3.x, (double)phi3.y);
         return phi3;
@@ -725,23 +739,26 @@ __device__ __forceinline__ cplx<T> mid_op_m(const PassArgs<T>& a, const ElemIn<T
         if (a.flags & F_RED_NORM) acc[0] += dens64((double)a.ph.gamma, (double)psi1.x, (double)psi1.y);
         if (a.flags & (F_NEXT_VSTEP1 | F_NEXT_PSI)) {
             if (a.flags & F_IMPRINT_NEXT) psi1 = apply_pin(a, psi1, e.c);
-            a.psiA[e.c.off] = psi1;
+            a.psiA[off] = psi1;
             if (a.flags & F_NEXT_VSTEP1) {
-                const T l1 = pot2(a, e.vx, psi1, false, (T)0);
-                return mul_exp(psi1, a.ph.c.x * l1, a.ph.c.y * l1);
+                // the next step's Vstep1, at time index it + 1
+                const cplx<T> m1 = modify_at<UH>(a, e, psi1, e.it + 1);
+                const T l1 = pot2<UH>(a, e.vx1, m1, false, (T)0, e.it + 1, off);
+                return mul_exp(m1, a.ph.c.x * l1, a.ph.c.y * l1);
             }
             return psi1;
         }
-        a.psiA[e.c.off] = psi1;
+        a.psiA[off] = psi1;
         return psi1;
     } else if constexpr (MM == MID_QF_VSTEP1) {
         // V_qf = Im(conj psi0 * F^-1[qf' k^2 F psi0]) / (rho0 + eps) (orginal 911-922), then Vstep1_qf
         const cplx<T> psi0 = e.aux;
         const T r0 = density(a.ph, psi0);
         const T q = (psi0.x * v.y + (-psi0.y) * v.x) / (r0 + (T)1.0e-12);
-        a.vq[e.c.off] = q;
-        const T lv = pot2(a, e.vx, psi0, true, q);
-        return mul_exp(psi0, a.ph.c.x * lv, a.ph.c.y * lv);
+        a.vq[off] = q;
+        const cplx<T> m0 = modify_at<UH>(a, e, psi0, e.it);
+        const T lv = pot2<UH>(a, e.vx, m0, true, q, e.it, off);
+        return mul_exp(m0, a.ph.c.x * lv, a.ph.c.y * lv);
     } else if constexpr (MM == MID_KIN_SUM) {
         const double k2 = a.kx64[e.c.ix] * a.kx64[e.c.ix] + a.ky64[e.c.iy] * a.ky64[e.c.iy] +
                           a.kz64[e.c.iz] * a.kz64[e.c.iz];
@@ -753,7 +770,7 @@ __device__ __forceinline__ cplx<T> mid_op_m(const PassArgs<T>& a, const ElemIn<T
         return v;
     } else if constexpr (MM == MID_OVERLAP_RE) {
         const cplx<T> p = e.aux;
-        a.rout[e.c.off] = (double)(p.x * v.x + p.y * v.y);
+        a.rout[off] = (double)(p.x * v.x + p.y * v.y);
         return v;
     } else if constexpr (MM == MID_EXPT || MM == MID_EXPTC || MM == MID_K2 || MM == MID_KA || MM == MID_VPP) {
         return mid_op_lean(a, e.c, v);
@@ -762,16 +779,16 @@ __device__ __forceinline__ cplx<T> mid_op_m(const PassArgs<T>& a, const ElemIn<T
     }
 }
 
-template <typename T>
+template <bool UH, typename T>
 __device__ __forceinline__ cplx<T> mid_op2(const PassArgs<T>& a, const ElemIn<T>& e, cplx<T> v, double* acc) {
     switch (a.mid_mode) {
-    case MID_VSTEP2: return mid_op_m<MID_VSTEP2>(a, e, v, acc);
-    case MID_PART1: return mid_op_m<MID_PART1>(a, e, v, acc);
-    case MID_VSTEP3: return mid_op_m<MID_VSTEP3>(a, e, v, acc);
-    case MID_QF_VSTEP1: return mid_op_m<MID_QF_VSTEP1>(a, e, v, acc);
-    case MID_KIN_SUM: return mid_op_m<MID_KIN_SUM>(a, e, v, acc);
-    case MID_VPP_SUM: return mid_op_m<MID_VPP_SUM>(a, e, v, acc);
-    case MID_OVERLAP_RE: return mid_op_m<MID_OVERLAP_RE>(a, e, v, acc);
+    case MID_VSTEP2: return mid_op_m<MID_VSTEP2, UH>(a, e, v, acc);
+    case MID_PART1: return mid_op_m<MID_PART1, UH>(a, e, v, acc);
+    case MID_VSTEP3: return mid_op_m<MID_VSTEP3, UH>(a, e, v, acc);
+    case MID_QF_VSTEP1: return mid_op_m<MID_QF_VSTEP1, UH>(a, e, v, acc);
+    case MID_KIN_SUM: return mid_op_m<MID_KIN_SUM, UH>(a, e, v, acc);
+    case MID_VPP_SUM: return mid_op_m<MID_VPP_SUM, UH>(a, e, v, acc);
+    case MID_OVERLAP_RE: return mid_op_m<MID_OVERLAP_RE, UH>(a, e, v, acc);
     case MID_EXPT:
     case MID_EXPTC:
     case MID_K2:
@@ -1130,28 +1147,34 @@ __device__ __forceinline__ const T* real_side(const PassArgs<T>& a) {
     return nullptr;
 }
 
-template <typename T, int N, bool CONTIG>
+template <bool UH, typename T, int N, bool CONTIG>
 __device__ __forceinline__ ElemIn<T> elem_in(const PassArgs<T>& a, const FullLds<T, N, CONTIG>& L, int w, int i,
-                                             bool sep, T vscale) {
+                                             bool sep, T vscale, unsigned it) {
     ElemIn<T> e;
+    e.it = it;
     const unsigned ui = (unsigned)i;
     if (a.axis == 2) { e.c.ix = L.c0[w]; e.c.iy = L.c1[w]; e.c.iz = ui; e.c.off = L.base[w] + ui; }
     else if (a.axis == 1) { e.c.ix = L.c0[w]; e.c.iy = ui; e.c.iz = L.c1[w]; e.c.off = L.base[w] + ui * (unsigned)a.ph.nz; }
     else { e.c.ix = ui; e.c.iy = L.c0[w]; e.c.iz = L.c1[w]; e.c.off = L.base[w] + ui * (unsigned)(a.ph.ny * a.ph.nz); }
     if constexpr (CONTIG) {
-        e.vx = sep ? vscale * (L.vxy[w] + L.vz[i]) : vext(a.ph, e.c.ix, e.c.iy, e.c.iz, e.c.off);
+        e.vx = sep ? vscale * (L.vxy[w] + L.vz[i]) : vext<UH>(a.ph, e.c.ix, e.c.iy, e.c.iz, e.c.off, it);
+        e.vx1 = e.vx;
+        if constexpr (UH) {
+            if (a.ph.vext == VEXT_USER) e.vx1 = vext<UH>(a.ph, e.c.ix, e.c.iy, e.c.iz, e.c.off, it + 1);
+        }
         const int li = lidx<N, FullLds<T, N, CONTIG>::W, CONTIG>(w, i);
         e.aux = L.aux[li];
         e.raux = L.raux[li];
     } else {
         e.vx = (T)0;
+        e.vx1 = (T)0;
         e.aux = mk((T)0, (T)0);
         e.raux = (T)0;
     }
     return e;
 }
 
-template <typename T, int N, bool CONTIG, int S1, int S2>
+template <typename T, int N, bool CONTIG, int S1, int S2, bool UH>
 __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CONTIG>& L, const TwE<T>* tw, double* acc) {
     constexpr int W = FullLds<T, N, CONTIG>::W;
     constexpr int TILE = W * N;
@@ -1161,6 +1184,7 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
     constexpr int RL = P.r[NSG - 1];
     constexpr int PERL = (W * (N / RL) + kThreads - 1) / kThreads;
     const unsigned p0 = blockIdx.x * (unsigned)W;
+    const unsigned it = time_index(a.ph);
 
     // per-pencil coordinates and the pencil part of a separable potential
     const int vk = a.ph.vext;
@@ -1176,7 +1200,7 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
         L.c1[w] = pc.c1;
         if (sep) {
             const int nx = a.ph.nx, ny = a.ph.ny;
-            if (vk == VEXT_WOODS_SAXON) L.vxy[w] = vext(a.ph, pc.c0, pc.c1, 0u, 0u);
+            if (vk == VEXT_WOODS_SAXON) L.vxy[w] = vext(a.ph, pc.c0, pc.c1, 0u, 0u, 0u);
             else L.vxy[w] = a.ph.vtab[pc.c0] + a.ph.vtab[nx + pc.c1];
             (void)ny;
         }
@@ -1225,7 +1249,7 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
                 else { i = e / W; w = e % W; }
                 if (L.ok[w]) {
                     const int li = lidx<N, W, CONTIG>(w, i);
-                    L.main[li] = load_op2(a, elem_in(a, L, w, i, sep, vscale), L.main[li], acc);
+                    L.main[li] = load_op2<UH>(a, elem_in<UH>(a, L, w, i, sep, vscale, it), L.main[li], acc);
                 }
             }
         }
@@ -1248,7 +1272,7 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
             else { i = e / W; w = e % W; }
             if (L.ok[w]) {
                 const int li = lidx<N, W, CONTIG>(w, i);
-                L.main[li] = mid_op2(a, elem_in(a, L, w, i, sep, vscale), L.main[li], acc);
+                L.main[li] = mid_op2<UH>(a, elem_in<UH>(a, L, w, i, sep, vscale, it), L.main[li], acc);
             }
         }
     }
@@ -1270,7 +1294,8 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
     }
 }
 
-template <typename T, int N, bool CONTIG, bool FULL, int SHAPE, int WD = 1, int MK = -1, bool CJ = false, int AX = -1>
+template <typename T, int N, bool CONTIG, bool FULL, int SHAPE, int WD = 1, int MK = -1, bool CJ = false, int AX = -1,
+          bool UH = false>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
 void fft_pass_kernel(const PassArgs<T> a) {
     __shared__ TwE<T> tw[N];
@@ -1294,7 +1319,7 @@ void fft_pass_kernel(const PassArgs<T> a) {
         __shared__ FullLds<T, N, CONTIG> L;
         __shared__ double red[kThreads / 64];
         double acc[3] = {0.0, 0.0, 0.0};
-        full_body<T, N, CONTIG, S1, S2>(a, L, tw, acc);
+        full_body<T, N, CONTIG, S1, S2, UH>(a, L, tw, acc);
         // FP64 partial sums of this workgroup (indices fixed at compile time)
         if (a.nred > 0) {
             const double s0 = block_sum(acc[0], red);

@@ -44,7 +44,17 @@ unsigned launch_fft_pass_n_impl(const PassArgs<T>& a, hipStream_t s) {
     const bool has_mid = lean && mid_kind(a, mk, cj);
     // lean mid ops live in the turn-around of the x axis
     if (lean && has_mid && (shape != SHAPE_FI || a.axis != 0)) return 0;
-    if (a.axis == 2) {
+    if (a.axis == 2 && a.ph.modify && !lean) {
+        // the physics z passes of engines with user hooks (INTERACTIONS / VEXT 100)
+#define GPE_LAUNCH_UH(SHAPE)                                                                              \
+    hipLaunchKernelGGL((fft_pass_kernel<T, NN, true, true, SHAPE, 1, -1, false, -1, true>), dim3(blocks), \
+                       dim3(kThreads), 0, s, a)
+        if (shape == SHAPE_F) GPE_LAUNCH_UH(SHAPE_F);
+        else if (shape == SHAPE_I) GPE_LAUNCH_UH(SHAPE_I);
+        else if (shape == SHAPE_IF) GPE_LAUNCH_UH(SHAPE_IF);
+        else return 0;
+#undef GPE_LAUNCH_UH
+    } else if (a.axis == 2) {
         if (lean && shape == SHAPE_F) GPE_LAUNCH(true, false, SHAPE_F, 1);
         else if (lean && shape == SHAPE_I) GPE_LAUNCH(true, false, SHAPE_I, 1);
         else if (shape == SHAPE_F) GPE_LAUNCH(true, true, SHAPE_F, 1);

@@ -79,6 +79,10 @@ void launch_from_c128(const cplx<double>* in, cplx<T>* out, unsigned n, hipStrea
 template <typename T>
 void launch_to_c128(const cplx<T>* in, cplx<double>* out, unsigned n, hipStream_t s);
 
+// the device time index of the steps of a run with user hooks: set it, advance it by one
+void launch_set_counter(unsigned* c, unsigned v, hipStream_t s);
+void launch_tick(unsigned* c, hipStream_t s);
+
 // plain device-to-device copy kernel of `bytes` (a multiple of 16)
 void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 

@@ -42,6 +42,16 @@ void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
     hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, (uint4*)dst, (const uint4*)src, n16);
 }
 
+// the step counter read by the passes of a run with user hooks: set once per
+// run (outside any captured graph), advanced by one after every step
+__global__ void set_counter_kernel(unsigned* c, unsigned v) { *c = v; }
+__global__ void tick_kernel(unsigned* c) { *c = *c + 1u; }
+
+void launch_set_counter(unsigned* c, unsigned v, hipStream_t s) {
+    hipLaunchKernelGGL(set_counter_kernel, dim3(1), dim3(1), 0, s, c, v);
+}
+void launch_tick(unsigned* c, hipStream_t s) { hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(1), 0, s, c); }
+
 void launch_finalize(const double* partials, int nb, int nq, double* scal, int mode, int slot, double npart,
                      double dt, double div, hipStream_t s) {
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, s, partials, nb, nq, scal, mode, slot, npart,

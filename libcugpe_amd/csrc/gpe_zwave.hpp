@@ -162,7 +162,7 @@ __device__ __forceinline__ void dedfdn_chunk(const Phys<T>& p, const cplx<T> (&v
         for (int j = 0; j < CH; j++) out[j] = (T)0;
     } else {
 #pragma unroll
-        for (int j = 0; j < CH; j++) out[j] = dedfdn(p, density(p, v[j]));
+        for (int j = 0; j < CH; j++) out[j] = dedfdn(p, density(p, v[j]), 0u, 0u);
     }
 }
 
@@ -284,6 +284,7 @@ void zwave_kernel(const PassArgs<T> a) {
     const unsigned base = pp * (unsigned)N + (unsigned)l;
     const T vscale = (vk == VEXT_COSINE) ? a.ph.up[0] : (T)1;
     const T vxy = sep ? a.ph.vtab[c.ix] + a.ph.vtab[a.ph.nx + c.iy] : (T)0;
+    const unsigned it = time_index(a.ph);
     double acc[3] = {0.0, 0.0, 0.0};
     auto off_of = [&](int b) __attribute__((always_inline)) { return base + (unsigned)(A * b); };
 
@@ -291,9 +292,11 @@ void zwave_kernel(const PassArgs<T> a) {
     auto elem_at = [&](int b) __attribute__((always_inline)) {
         ElemIn<T> e;
         e.c = c;
+        e.it = it;
         e.c.iz = (unsigned)(l + A * b);
         e.c.off = base + (unsigned)(A * b);
-        e.vx = sep ? vscale * (vxy + L.vz[l + A * b]) : vext(a.ph, e.c.ix, e.c.iy, e.c.iz, e.c.off);
+        e.vx = sep ? vscale * (vxy + L.vz[l + A * b]) : vext(a.ph, e.c.ix, e.c.iy, e.c.iz, e.c.off, it);
+        e.vx1 = e.vx;
         e.aux = mk((T)0, (T)0);
         e.raux = (T)0;
         return e;
@@ -329,7 +332,7 @@ void zwave_kernel(const PassArgs<T> a) {
                               MM == MID_QF_VSTEP1 || MM == MID_OVERLAP_RE;
     const T* rs = real_side(a);
     const bool qf = (a.flags & F_QF) != 0;
-    // the fast chunk forms: ops without imprint
+    // the fast chunk forms: ops without imprint, built-in physics
     const bool fast = (MM == MID_VSTEP2) || (MM == MID_VSTEP3 && !(a.flags & F_IMPRINT_NEXT)) ||
                       (LM == LOAD_PSI0 && !(a.flags & F_IMPRINT)) || (LM == LOAD_PART2 && !(a.flags & F_IMPRINT_MID)) ||
                       MM == MID_PART1 || MM == MID_QF_VSTEP1;
@@ -357,7 +360,7 @@ void zwave_kernel(const PassArgs<T> a) {
 #pragma unroll
             for (int j = 0; j < CH; j++) {
                 const int b = q * CH + j;
-                vx[j] = vext(a.ph, c.ix, c.iy, (unsigned)(l + A * b), base + (unsigned)(A * b));
+                vx[j] = vext(a.ph, c.ix, c.iy, (unsigned)(l + A * b), base + (unsigned)(A * b), it);
             }
         }
     };
@@ -539,7 +542,9 @@ unsigned launch_zwave(const PassArgs<T>& a, hipStream_t s) {
     if constexpr (zw_lanes<T>(N) == 0) {
         return 0;
     } else {
-        if (a.axis != 2 || a.nred > 2) return 0;
+        // engines with user hooks run their z passes in the LDS-tile kernels
+        // compiled with the hook file; these kernels carry none of its code
+        if (a.axis != 2 || a.nred > 2 || a.ph.modify) return 0;
         const unsigned blocks = (a.M + ZW<T, N>::PENCILS - 1) / ZW<T, N>::PENCILS;
         const int shape = pass_shape(a.dir1, a.dir2);
         const int lm = (a.load_mode == LOAD_PLAIN && (a.flags & F_RED_POT)) ? LOAD_REDPOT : a.load_mode;

@@ -15,9 +15,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgpe_hip.so")
 
-# physics codes (include/gpe_user_defined.h)
+# physics codes (include/gpe_user_defined.h); USER selects the user hook file
 UFG, ETF, CONTACT, DIPOLAR = 0, 1, -1, -2
 HARMONIC, WOODS_SAXON, COSINE, VARRAY = 0, 1, 2, 3
+USER = 100
 PARTICLES, DIMERS = 1, 2
 C128, C64 = 128, 64
 
@@ -93,6 +94,7 @@ def lib():
             "gpe_ensemble_set_quantum_friction_coeff": [_p, _d],
             "gpe_ensemble_set_psi": [_p, _i, _p],
             "gpe_ensemble_set_external_potential": [_p, _i, _p],
+            "gpe_ensemble_set_instance_params": [_p, _i, _i, _p],
             "gpe_ensemble_get_psi": [_p, _i, _dp, _p],
             "gpe_ensemble_evolve": [_p, _i],
             "gpe_ensemble_set_graph": [_p, _i],
@@ -377,6 +379,11 @@ class Ensemble:
 
     def set_quantum_friction_coeff(self, qf):
         _check(lib().gpe_ensemble_set_quantum_friction_coeff(self.h, qf), "gpe_ensemble_set_quantum_friction_coeff")
+
+    def set_instance_params(self, inst, params):
+        """User parameters of one lattice (read by the user hooks as its d_user_param)."""
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64))
+        _check(lib().gpe_ensemble_set_instance_params(self.h, inst, len(p), _ptr(p)), "gpe_ensemble_set_instance_params")
 
     def set_external_potential(self, inst, v):
         a = np.ascontiguousarray(np.asarray(v, dtype=np.float64))

@@ -17,6 +17,7 @@
 #include <omp.h>
 
 #include "gpe_oracle.h"
+#include "user_hooks.h"
 
 typedef struct { double re, im; } cd;
 
@@ -52,14 +53,27 @@ struct gpeo {
 
 /* ------------------------------------------------------------------ hooks */
 
-/* gpe_external_potential: VEXT 0 harmonic (orginal/gpe_user_defined.h:79-101,
- * dev/gpe_user_defined.h:488-513), VEXT 1 Woods-Saxon tube
- * (dev/gpe_user_defined.h:533-554), VEXT 2 cosine lattice and VEXT 3 array
- * (additions of this engine). */
-static inline double vext_val(const gpeo_t *e, unsigned ix, unsigned iy, unsigned iz, size_t ixyz)
+#define USER_CODE 100 /* INTERACTIONS / VEXT of the user hook file */
+
+/* the user hook file's view of this engine */
+static inline gpeo_uh_ctx uh_ctx(const gpeo_t *e)
+{
+    gpeo_uh_ctx c = {e->up, e->nx, e->ny, e->nz, e->dt, e->t0, e->npart};
+    return c;
+}
+
+/* gpe_external_potential at time index it: VEXT 0 harmonic
+ * (orginal/gpe_user_defined.h:79-101, dev/gpe_user_defined.h:488-513), VEXT 1
+ * Woods-Saxon tube (dev/gpe_user_defined.h:533-554), VEXT 2 cosine lattice and
+ * VEXT 3 array (additions of this engine), VEXT 100 the user hook file. */
+static inline double vext_val(const gpeo_t *e, unsigned ix, unsigned iy, unsigned iz, size_t ixyz, unsigned it)
 {
     const double *p = e->up;
     switch (e->vext) {
+    case USER_CODE: {
+        gpeo_uh_ctx c = uh_ctx(e);
+        return gpeo_uh_vext(&c, ix, iy, iz, it);
+    }
     case 0: {
         double x = (double)ix - 1.0 * (e->nx / 2);
         double y = (double)iy - 1.0 * (e->ny / 2);
@@ -87,12 +101,17 @@ static inline double vext_val(const gpeo_t *e, unsigned ix, unsigned iy, unsigne
 #define XI_ 0.37
 #define CONTACT_ 0.901
 
-/* gpe_dEDFdn: INTERACTIONS 0 unitary Fermi gas (orginal/gpe_user_defined.h:122-126,
- * dev/gpe_user_defined.h:131-135), 1 ETF BEC side (dev 214-243), -1 contact
- * bosons (dev 317-322), -2 dipolar local part (dev 400-404). */
-static inline double dedfdn(const gpeo_t *e, double rho)
+/* gpe_dEDFdn at time index it: INTERACTIONS 0 unitary Fermi gas
+ * (orginal/gpe_user_defined.h:122-126, dev/gpe_user_defined.h:131-135), 1 ETF
+ * BEC side (dev 214-243), -1 contact bosons (dev 317-322), -2 dipolar local
+ * part (dev 400-404), 100 the user hook file. */
+static inline double dedfdn(const gpeo_t *e, double rho, unsigned it)
 {
     switch (e->interactions) {
+    case USER_CODE: {
+        gpeo_uh_ctx c = uh_ctx(e);
+        return gpeo_uh_dedfdn(&c, rho, it);
+    }
     case 0:
         return 0.37 * pow(3.0 * M_PI * M_PI * rho, 2.0 / 3.0) / 2.0;
     case 1: {
@@ -119,9 +138,13 @@ static inline double dedfdn(const gpeo_t *e, double rho)
 
 /* gpe_EDF: orginal/gpe_user_defined.h:109-114, dev/gpe_user_defined.h:118-123,
  * 196-206, 305-309, 387-392 */
-static inline double edf(const gpeo_t *e, double rho)
+static inline double edf(const gpeo_t *e, double rho, unsigned it)
 {
     switch (e->interactions) {
+    case USER_CODE: {
+        gpeo_uh_ctx c = uh_ctx(e);
+        return gpeo_uh_edf(&c, rho, it);
+    }
     case 0:
         return 0.37 * 0.6 * rho * pow(3.0 * M_PI * M_PI * rho, 2.0 / 3.0) / 2.;
     case 1: {
@@ -144,8 +167,15 @@ static inline double edf(const gpeo_t *e, double rho)
 /* gpe_density: orginal/gpe_engine.cu:384-387 */
 static inline double density(const gpeo_t *e, cd p) { return e->gamma * (p.re * p.re + p.im * p.im); }
 
-/* gpe_modify_psi: identity (orginal/gpe_user_defined.h:66-69) */
-static inline cd modify_psi(cd p) { return p; }
+/* gpe_modify_psi at time index it: the identity of orginal/gpe_user_defined.h:66-69,
+ * or the user hook file's when the physics comes from it */
+static inline cd modify_psi(const gpeo_t *e, unsigned ix, unsigned iy, unsigned iz, unsigned it, cd p)
+{
+    if (e->interactions != USER_CODE && e->vext != USER_CODE) return p;
+    gpeo_uh_ctx c = uh_ctx(e);
+    gpeo_uh_modify(&c, ix, iy, iz, it, &p.re, &p.im);
+    return p;
+}
 
 /* ---------------------------------------------------------- utilities */
 
@@ -387,9 +417,9 @@ static void vstep1(gpeo_t *e, unsigned it, const cd *psi_in, cd *psi_out, double
         for (int iy = 0; iy < e->ny; iy++)
             for (int iz = 0; iz < e->nz; iz++) {
                 size_t i = IDX3(ix, iy, iz);
-                cd l = modify_psi(psi_in[i]);
+                cd l = modify_psi(e, ix, iy, iz, it, psi_in[i]);
                 double r = density(e, l);
-                double lv = vext_val(e, ix, iy, iz, i) + dedfdn(e, r);
+                double lv = vext_val(e, ix, iy, iz, i, it) + dedfdn(e, r, it);
                 if (qf) lv = lv + qf[i];
                 wrkR[i] = lv;
                 cd ex = cexp_(cscale(e->c, lv));
@@ -410,10 +440,10 @@ static void vstep2(gpeo_t *e, unsigned it, cd *psi_in, cd *psi_out, const double
                 cd ex = cexp_(cscale(e->c, lv));
                 l = cmul(l, ex);
                 double r = density(e, l);
-                lv = 0.5 * (lv + vext_val(e, ix, iy, iz, i) + dedfdn(e, r));
+                lv = 0.5 * (lv + vext_val(e, ix, iy, iz, i, it + 1) + dedfdn(e, r, it + 1));
                 ex = cexp_(cscale(e->c, lv));
                 wrkC[i] = ex;
-                l = modify_psi(psi_out[i]);
+                l = modify_psi(e, ix, iy, iz, it, psi_out[i]);
                 psi_out[i] = cmul(l, ex);
             }
 }
@@ -437,11 +467,11 @@ static void vstep2_part2(gpeo_t *e, unsigned it, cd *psi_in, cd *psi_out, const 
                 cd l = psi_in[i];
                 double lv = wrkR[i];
                 double r = density(e, l);
-                if (qf) lv = 0.5 * (lv + vext_val(e, ix, iy, iz, i) + dedfdn(e, r) + qf[i]);
-                else lv = 0.5 * (lv + vext_val(e, ix, iy, iz, i) + dedfdn(e, r));
+                if (qf) lv = 0.5 * (lv + vext_val(e, ix, iy, iz, i, it + 1) + dedfdn(e, r, it + 1) + qf[i]);
+                else lv = 0.5 * (lv + vext_val(e, ix, iy, iz, i, it + 1) + dedfdn(e, r, it + 1));
                 cd ex = cexp_(cscale(e->c, lv));
                 wrkC[i] = ex;
-                l = modify_psi(psi_out[i]);
+                l = modify_psi(e, ix, iy, iz, it, psi_out[i]);
                 psi_out[i] = cmul(l, ex);
             }
 }
@@ -633,8 +663,8 @@ int gpeo_energy(gpeo_t *e, double *t, double *ekin, double *eint, double *eext)
             for (int iz = 0; iz < e->nz; iz++) {
                 size_t i = IDX3(ix, iy, iz);
                 double r = density(e, e->psi[i]);
-                se += r * vext_val(e, ix, iy, iz, i);
-                si += edf(e, r);
+                se += r * vext_val(e, ix, iy, iz, i, e->it);
+                si += edf(e, r, e->it);
             }
         pl_ext[ix] = se;
         pl_int[ix] = si;
@@ -722,14 +752,14 @@ int gpeo_tables(const gpeo_t *e, double *kx, double *ky, double *kz, double *Kx,
 
 int gpeo_vext_value(const gpeo_t *e, int ix, int iy, int iz, double *v)
 {
-    *v = vext_val(e, (unsigned)ix, (unsigned)iy, (unsigned)iz, IDX3(ix, iy, iz));
+    *v = vext_val(e, (unsigned)ix, (unsigned)iy, (unsigned)iz, IDX3(ix, iy, iz), e->it);
     return 0;
 }
 
 int gpeo_dedfdn_value(const gpeo_t *e, double rho, double *dv, double *edfv)
 {
-    if (dv) *dv = dedfdn(e, rho);
-    if (edfv) *edfv = edf(e, rho);
+    if (dv) *dv = dedfdn(e, rho, e->it);
+    if (edfv) *edfv = edf(e, rho, e->it);
     return 0;
 }
 
@@ -829,7 +859,7 @@ static double dgdrho(const gpeo_t *e, double rho)
 {
     switch (e->interactions) {
     case 0: /* g = 0.37 (3 pi^2 rho)^(2/3) / 2  ->  g' = (2/3) g / rho */
-        return rho > 0.0 ? (2.0 / 3.0) * dedfdn(e, rho) / rho : 0.0;
+        return rho > 0.0 ? (2.0 / 3.0) * dedfdn(e, rho, 0) / rho : 0.0;
     case -1:
         return 4 * M_PI * e->up[3];
     case -2:
@@ -872,7 +902,7 @@ static void backward_step(gpeo_t *e, const cd *psi0, const cd *psi1, cd *lam, do
         for (int iy = 0; iy < e->ny; iy++)
             for (int iz = 0; iz < e->nz; iz++) {
                 size_t i = IDX3(ix, iy, iz);
-                double V0 = vext_val(e, ix, iy, iz, i) + dedfdn(e, density(e, psi0[i]));
+                double V0 = vext_val(e, ix, iy, iz, i, 0) + dedfdn(e, density(e, psi0[i]), 0);
                 phi[i] = cmul(psi0[i], cexp_(cscale(c, V0)));
             }
     kinetic_step(e, phi, 0); /* phi2 */
@@ -881,10 +911,10 @@ static void backward_step(gpeo_t *e, const cd *psi0, const cd *psi1, cd *lam, do
         for (int iy = 0; iy < e->ny; iy++)
             for (int iz = 0; iz < e->nz; iz++) {
                 size_t i = IDX3(ix, iy, iz);
-                double vx = vext_val(e, ix, iy, iz, i);
-                double V0 = vx + dedfdn(e, density(e, psi0[i]));
+                double vx = vext_val(e, ix, iy, iz, i, 0);
+                double V0 = vx + dedfdn(e, density(e, psi0[i]), 0);
                 cd phi3 = cmul(phi[i], cexp_(cscale(c, V0)));
-                double V1 = 0.5 * (V0 + vx + dedfdn(e, density(e, phi3)));
+                double V1 = 0.5 * (V0 + vx + dedfdn(e, density(e, phi3), 0));
                 cd W = cexp_(cscale(c, V1));
                 vb[i] = sens(lam[i], psi1[i], c);
                 lam[i] = cmul(cconj(W), lam[i]);
@@ -896,12 +926,12 @@ static void backward_step(gpeo_t *e, const cd *psi0, const cd *psi1, cd *lam, do
         for (int iy = 0; iy < e->ny; iy++)
             for (int iz = 0; iz < e->nz; iz++) {
                 size_t i = IDX3(ix, iy, iz);
-                double vx = vext_val(e, ix, iy, iz, i);
-                double V0 = vx + dedfdn(e, density(e, psi0[i]));
+                double vx = vext_val(e, ix, iy, iz, i, 0);
+                double V0 = vx + dedfdn(e, density(e, psi0[i]), 0);
                 cd E0 = cexp_(cscale(c, V0));
                 cd phi3 = phi[i];
                 double r3 = density(e, phi3);
-                double V1 = 0.5 * (V0 + vx + dedfdn(e, r3));
+                double V1 = 0.5 * (V0 + vx + dedfdn(e, r3, 0));
                 cd W = cexp_(cscale(c, V1));
                 cd chi = cmul(psi0[i], W);
                 lam0[i] = cmul(cconj(W), lam[i]);
@@ -920,7 +950,7 @@ static void backward_step(gpeo_t *e, const cd *psi0, const cd *psi1, cd *lam, do
             for (int iz = 0; iz < e->nz; iz++) {
                 size_t i = IDX3(ix, iy, iz);
                 double r0 = density(e, psi0[i]);
-                double V0 = vext_val(e, ix, iy, iz, i) + dedfdn(e, r0);
+                double V0 = vext_val(e, ix, iy, iz, i, 0) + dedfdn(e, r0, 0);
                 cd E0 = cexp_(cscale(c, V0));
                 cd phi1 = cmul(psi0[i], E0);
                 cd l0 = lam0[i];
@@ -939,7 +969,7 @@ static void backward_step(gpeo_t *e, const cd *psi0, const cd *psi1, cd *lam, do
 int gpeo_adjoint_overlap(gpeo_t *e, int nt, const double *psi_target, int ckpt, double *loss, double *grad)
 {
     if (nt < 1 || !psi_target || !grad) return -12;
-    if (e->beta != 0.0 || e->qf != 0.0 || isnan(dgdrho(e, 1.0))) return -12;
+    if (e->beta != 0.0 || e->qf != 0.0 || e->vext == USER_CODE || isnan(dgdrho(e, 1.0))) return -12;
     const size_t n = e->n;
     const int C = ckpt > 0 ? ckpt : (int)ceil(sqrt((double)nt));
     const int nck = (nt + C - 1) / C;
@@ -1054,9 +1084,9 @@ static void eulerstrang_vstep(gpeo_t *e, cd *psi, const cd *vdip)
         for (int iy = 0; iy < e->ny; iy++)
             for (int iz = 0; iz < e->nz; iz++) {
                 size_t i = IDX3(ix, iy, iz);
-                cd l = modify_psi(psi[i]);
+                cd l = modify_psi(e, ix, iy, iz, e->it, psi[i]);
                 double r = density(e, l);
-                double lv = vext_val(e, ix, iy, iz, i) + dedfdn(e, r) + vdip[i].re;
+                double lv = vext_val(e, ix, iy, iz, i, e->it) + dedfdn(e, r, e->it) + vdip[i].re;
                 psi[i] = cmul(l, cexp_(cscale(e->c, lv)));
             }
 }

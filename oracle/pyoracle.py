@@ -25,9 +25,18 @@ _dp = ctypes.POINTER(ctypes.c_double)
 
 
 def build(force=False):
-    """Compile oracle/liboracle.so with the committed Makefile."""
+    """Compile oracle/liboracle.so with the committed Makefile (make's own
+    dependency check: the sources and the user hook file).  Where no
+    compiler is present, as on a box that only runs the prebuilt library, an
+    existing library is used as is."""
     if force or not os.path.exists(_LIB_PATH):
         subprocess.check_call(["make", "-s", "-C", _HERE, "-B" if force else "all"])
+    else:
+        try:
+            subprocess.run(["make", "-s", "-C", _HERE, "all"], check=True, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL)
+        except (OSError, subprocess.CalledProcessError):
+            pass
     return _LIB_PATH
 
 
