@@ -34,9 +34,14 @@ DT = W.HEADLINE_DT
 initial_state = W.headline_state
 
 
+VARIANT = None  # launcher variant (--variant); None keeps the engine's default choice
+
+
 def make_engine(E, shape, precision, psi):
     n = int(np.prod(shape))
     eng = E.Engine(shape, 1.0, 0.0, DT, float(n), E.CONTACT, E.COSINE, 0, precision)
+    if VARIANT is not None:
+        eng.set_variant(VARIANT)
     eng.set_user_params(W.HEADLINE_PARAMS)
     eng.set_psi(psi)
     eng.normalize_psi()
@@ -435,7 +440,11 @@ def main():
     ap.add_argument("--ufg-oracle-steps", type=int, default=40)
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-oracle-steps", type=int, default=200)
+    ap.add_argument("--variant", type=int, default=None, help="launcher variant of the 256^3 engines (bits: 1 z in "
+                    "LDS tiles, 2 half-width strided tiles, 4 pipelined strided passes)")
     a = ap.parse_args()
+    global VARIANT
+    VARIANT = a.variant
     if a.gpus != 1:
         print("bench.py: this engine runs on one GPU; a %d-GPU run is not supported "
               "(one lattice is not sharded across GPUs)" % a.gpus, file=sys.stderr)

@@ -219,7 +219,9 @@ const char *gpe_build_info(void);
 /* Launcher choice of an engine, picked by timing (no counterpart in the
  * reference, whose transforms are cuFFT plans): bit 0 runs the z passes in
  * LDS tiles instead of wave registers, bit 1 halves the tile width of the
- * strided passes.  autotune times `steps` real-time steps per choice on the
+ * strided passes, bit 2 runs the strided passes of complex64 engines in
+ * persistent workgroups that copy the next tile into LDS while transforming
+ * the current one.  autotune times `steps` real-time steps per choice on the
  * current state (restored afterwards) and keeps the fastest. */
 int gpe_engine_autotune(gpe_engine_t e, int steps, int *variant);
 int gpe_engine_set_variant(gpe_engine_t e, int variant);

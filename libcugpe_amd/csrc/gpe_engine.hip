@@ -49,6 +49,7 @@ struct gpe_engine_s {
     size_t n = 0;        // points of one lattice
     size_t batch = 1;    // lattices of this shape stored one after another (ensembles)
     int variant = 0;     // kernel choice of the pass launcher (autotuned)
+    int cus = 256;       // compute units of the device (persistent pass grids)
     int precision = GPE_PRECISION_C128;
     int interactions = 0, vext = 0, gpe_for = 0;
     double gamma = 2.0;
@@ -322,6 +323,7 @@ int run_pass(gpe_engine_s* e, PassArgs<T> a, const PassSpec& s) {
     const int N = axis_len(e, s.axis);
     a.M = (unsigned)(e->n * e->batch / (size_t)N);
     a.variant = e->variant;
+    a.pipe_wgs = 2u * (unsigned)e->cus;
     a.n1 = (unsigned)e->n;
     a.src = (const cplx<T>*)s.src;
     a.dst = (cplx<T>*)s.dst;
@@ -720,6 +722,12 @@ int create_impl(gpe_engine_s** out, const gpe_config_t* cfg, double alpha, doubl
     int r = (e->precision == GPE_PRECISION_C64) ? upload_static_tables<float>(e) : upload_static_tables<double>(e);
     if (!r) r = upload_potential_any(e);
     if (r) return fail(r);
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            e->cus = cus;
+    }
     e->variant = default_variant(e);
     *out = e;
     return 0;
@@ -1620,7 +1628,8 @@ int autotune_t(gpe_engine_s* e, int steps, int* chosen) {
     int err = 0;
     (void)hipMemcpyAsync(save, e->A, bytes, hipMemcpyDeviceToDevice, e->stream);
     const double beta = e->beta;
-    for (int v = 0; v < 4 && !err; v++) {
+    for (int v = 0; v < 8 && !err; v++) {
+        if ((v & 4) && e->precision != GPE_PRECISION_C64) continue;  // pipelined passes: complex64 only
         e->variant = v;
         StepPlan sp;
         // one untimed step, then the timed ones (real-time schedule in both time modes)
@@ -1672,7 +1681,7 @@ int gpe_engine_set_graph(gpe_engine_t e, int on) {
 int gpe_engine_set_variant(gpe_engine_t e, int variant) {
     if (!e) return GPE_ERR_NO_ENGINE;
     e->gen++;
-    if (variant < 0 || variant > 3) return GPE_ERR_ARG;
+    if (variant < 0 || variant > 7) return GPE_ERR_ARG;
     e->variant = variant;
     return 0;
 }

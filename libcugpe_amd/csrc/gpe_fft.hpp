@@ -431,7 +431,9 @@ struct PassArgs {
     int dir1, dir2;        // -1 / +1; dir2 == 0: single transform
     int load_mode, mid_mode, store_mode, flags;
     int nred;              // number of FP64 sums produced (0..3)
-    int variant;           // launcher choice: bit 0 z passes in LDS tiles, bit 1 half-width strided tiles
+    int variant;           // launcher choice: bit 0 z passes in LDS tiles, bit 1 half-width strided tiles,
+                           // bit 2 pipelined strided passes (complex64)
+    unsigned pipe_wgs;     // persistent workgroups of a pipelined pass (2 per CU)
     const cplx<T>* src;
     cplx<T>* dst;
     const cplx<T>* tw;     // exp(-2 pi i m / N), m < N
@@ -1057,6 +1059,36 @@ constexpr int lean_w() {
     return tile_w<T, N>() / WD > 0 ? tile_w<T, N>() / WD : 1;
 }
 
+// What follows the first stage of a lean pass of two or more stages (the first
+// stage's output is in LDS): the remaining stages, the mid op, the optional
+// second transform and the store of tile p0.
+template <typename T, int N, int W, bool CONTIG, int S1, int S2, int MK, bool CJ, int AX>
+__device__ __forceinline__ void lean_tail(const PassArgs<T>& a, unsigned p0, cplx<T>* lds, const TwE<T>* tw) {
+    constexpr FftPlan P = plan_of(N, false, sizeof(T) == 4);
+    constexpr int NSG = P.ns;
+    constexpr int RL = P.r[NSG - 1];
+    constexpr int PERL = (W * (N / RL) + kThreads - 1) / kThreads;
+    cplx<T> v[PERL][RL];
+    lds_stages<T, N, W, CONTIG, S1, false, 1, NSG - 1>(lds, tw);
+    last_stage_regs<T, N, W, CONTIG, S1, RL, PERL>(lds, tw, v);
+    if constexpr (MK >= 0) mid_regs_k<MK, CJ, T, N, W, CONTIG, RL, PERL, AX>(a, p0, v);
+    if constexpr (S2 != 0) {
+        constexpr FftPlan Q = plan_of(N, true, sizeof(T) == 4);
+        static_assert(Q.r[0] == RL, "reversed plan must start with the last radix");
+        constexpr int RF = Q.r[NSG - 1];
+        constexpr int PERF = (W * (N / RF) + kThreads - 1) / kThreads;
+        __syncthreads();  // the first transform's last-stage reads are done
+        first_stage_regs<T, N, W, CONTIG, S2, RL, PERL>(v, lds);
+        __syncthreads();
+        lds_stages<T, N, W, CONTIG, S2, true, 1, NSG - 1>(lds, tw);
+        cplx<T> u[PERF][RF];
+        last_stage_regs<T, N, W, CONTIG, S2, RF, PERF>(lds, tw, u);
+        store_regs<T, N, W, CONTIG, RF, PERF, AX>(a, p0, u);
+    } else {
+        store_regs<T, N, W, CONTIG, RL, PERL, AX>(a, p0, v);
+    }
+}
+
 // MK: compile-time kind of the mid op (MidKind), -1 for none; CJ conjugates it
 template <typename T, int N, bool CONTIG, int S1, int S2, int WD, int MK, bool CJ, int AX>
 __device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, const TwE<T>* tw) {
@@ -1066,38 +1098,114 @@ __device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, co
     constexpr int RL = P.r[NSG - 1];
     constexpr int PERL = (W * (N / RL) + kThreads - 1) / kThreads;
     const unsigned p0 = blockIdx.x * (unsigned)W;
-    cplx<T> v[PERL][RL];
     if constexpr (NSG == 1) {
+        cplx<T> v[PERL][RL];
         load_regs<T, N, W, CONTIG, false, RL, PERL, AX>(a, p0, v, nullptr);
 #pragma unroll
         for (int q = 0; q < PERL; q++) dft<RL, S1>(v[q]);
+        if constexpr (MK >= 0) mid_regs_k<MK, CJ, T, N, W, CONTIG, RL, PERL, AX>(a, p0, v);
+        if constexpr (S2 != 0) {
+#pragma unroll
+            for (int q = 0; q < PERL; q++) dft<RL, S2>(v[q]);
+        }
+        store_regs<T, N, W, CONTIG, RL, PERL, AX>(a, p0, v);
     } else {
         first_stage_global<T, N, W, CONTIG, false, S1, P.r[0], AX>(a, p0, lds, nullptr);
         __syncthreads();
-        lds_stages<T, N, W, CONTIG, S1, false, 1, NSG - 1>(lds, tw);
-        last_stage_regs<T, N, W, CONTIG, S1, RL, PERL>(lds, tw, v);
+        lean_tail<T, N, W, CONTIG, S1, S2, MK, CJ, AX>(a, p0, lds, tw);
     }
-    if constexpr (MK >= 0) mid_regs_k<MK, CJ, T, N, W, CONTIG, RL, PERL, AX>(a, p0, v);
-    if constexpr (S2 != 0) {
-        if constexpr (NSG == 1) {
+}
+
+// ------------------------------------------------ pipelined strided passes
+//
+// complex64 lean passes along a strided axis (y or x) in persistent
+// workgroups: workgroup b transforms tiles b, b + gridDim.x, ...  The rows of
+// the next tile are copied from HBM straight into an LDS staging buffer
+// (global_load_lds_dwordx4: 64 lanes x 16 bytes, 8 rows of 16 columns per
+// instruction, no registers) as soon as the current tile's first stage has
+// read its own rows, so the next tile's loads are in flight while the
+// current tile's remaining stages, mid op, second transform and stores run.
+// Every row of a tile is W contiguous elements at base + i * stride (the
+// launcher checks the lattice allows it).
+
+// lengths with a pipelined kernel: two or more stages, a tile of at most 4096
+// elements (32 KB of staging) and whole 16-byte lane copies per row
+template <int N>
+constexpr bool pipe_ok() {
+    constexpr int W = lean_w<float, N, 1>();
+    if (W < 2 || W > 128 || W % 2 != 0 || N * W > 4096 || plan_of(N, false, true).ns < 2) return false;
+    return N % ((128 / W) * (kThreads / 64)) == 0;  // whole wave instructions of 128 / W rows per wave
+}
+
+// issues the copies of tile p0's N rows into stage ([row][W], linear)
+template <int N, int W, int AX>
+__device__ __forceinline__ void pipe_fetch(const PassArgs<float>& a, unsigned p0, cplx<float>* stage) {
+    constexpr int LPR = W / 2;          // lanes per row, 16 bytes (2 elements) each
+    constexpr int RPI = 64 / LPR;       // rows per wave instruction
+    constexpr int NI = N / RPI;         // instructions per tile
+    constexpr int PERW = NI / (kThreads / 64);
+    static_assert(PERW * (kThreads / 64) * RPI == N, "tile rows must split evenly over the waves");
+    const Pencil pc = make_pencil<AX>(a, N, p0);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const unsigned rr = (unsigned)(lane / LPR), cc = 2u * (unsigned)(lane % LPR);
 #pragma unroll
-            for (int q = 0; q < PERL; q++) dft<RL, S2>(v[q]);
-        } else {
-            constexpr FftPlan Q = plan_of(N, true, sizeof(T) == 4);
-            static_assert(Q.r[0] == RL, "reversed plan must start with the last radix");
-            constexpr int RF = Q.r[NSG - 1];
-            constexpr int PERF = (W * (N / RF) + kThreads - 1) / kThreads;
-            __syncthreads();  // the first transform's last-stage reads are done
-            first_stage_regs<T, N, W, CONTIG, S2, RL, PERL>(v, lds);
-            __syncthreads();
-            lds_stages<T, N, W, CONTIG, S2, true, 1, NSG - 1>(lds, tw);
-            cplx<T> u[PERF][RF];
-            last_stage_regs<T, N, W, CONTIG, S2, RF, PERF>(lds, tw, u);
-            store_regs<T, N, W, CONTIG, RF, PERF, AX>(a, p0, u);
-            return;
+    for (int q = 0; q < PERW; q++) {
+        const int g = wv * PERW + q;
+        const cplx<float>* src = a.src + pc.base + ((unsigned)(g * RPI) + rr) * pc.stride + cc;
+        __builtin_amdgcn_global_load_lds((const void*)src,
+                                         (__attribute__((address_space(3))) void*)(stage + g * 2 * 64), 16, 0, 0);
+    }
+}
+
+// the first Stockham stage from the staging buffer into the tile
+template <int N, int W, int S, int R>
+__device__ __forceinline__ void first_stage_stage(const cplx<float>* stage, cplx<float>* lds) {
+    constexpr int M = N / R;
+    constexpr int NB = W * M;
+    constexpr int PER = (NB + kThreads - 1) / kThreads;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int b = threadIdx.x + q * kThreads;
+        if ((NB % kThreads) == 0 || b < NB) {
+            int w, j;
+            bmap<M, W, false>(b, w, j);
+            cplx<float> v[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = stage[(j + r * M) * W + w];
+            dft<R, S>(v);
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[lidx<N, W, false>(w, j * R + r)] = v[r];
         }
     }
-    store_regs<T, N, W, CONTIG, RL, PERL, AX>(a, p0, v);
+}
+
+template <int N, int SHAPE, int MK, bool CJ, int AX>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
+void fft_pipe_kernel(const PassArgs<float> a) {
+    constexpr int W = lean_w<float, N, 1>();
+    constexpr FftPlan P = plan_of(N, false, true);
+    constexpr int S1 = (SHAPE == SHAPE_F || SHAPE == SHAPE_FI) ? -1 : 1;
+    constexpr int S2 = (SHAPE == SHAPE_FI) ? 1 : (SHAPE == SHAPE_IF) ? -1 : 0;
+    __shared__ TwE<float> tw[N];
+    __shared__ cplx<float> stage[W * N];
+    __shared__ cplx<float> lds[lds_elems<N, W, false>()];
+    for (int i = threadIdx.x; i < N; i += kThreads) {
+        const cplx<float> h = a.tw[i], lo = a.twlo[i];
+        tw[i].t = make_float4(h.x, h.y, lo.x, lo.y);
+    }
+    const unsigned ntiles = a.M / (unsigned)W;
+    unsigned t = blockIdx.x;
+    if (t < ntiles) pipe_fetch<N, W, AX>(a, t * (unsigned)W, stage);
+    for (; t < ntiles; t += gridDim.x) {
+        // this tile's rows have landed in the staging buffer (the wait also
+        // covers the previous tile's stores), for every wave of the group
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        first_stage_stage<N, W, S1, P.r[0]>(stage, lds);
+        __syncthreads();  // the staging buffer is free, the tile is in LDS
+        if (t + gridDim.x < ntiles) pipe_fetch<N, W, AX>(a, (t + gridDim.x) * (unsigned)W, stage);
+        lean_tail<float, N, W, false, S1, S2, MK, CJ, AX>(a, t * (unsigned)W, lds, tw);
+    }
 }
 
 // Physics passes.  Every per-element global input is gathered into LDS up

@@ -44,6 +44,38 @@ unsigned launch_fft_pass_n_impl(const PassArgs<T>& a, hipStream_t s) {
     const bool has_mid = lean && mid_kind(a, mk, cj);
     // lean mid ops live in the turn-around of the x axis
     if (lean && has_mid && (shape != SHAPE_FI || a.axis != 0)) return 0;
+    // pipelined persistent strided passes (variant bit 2, complex64): every
+    // tile must be W whole rows of W contiguous columns inside one lattice
+    if constexpr (sizeof(T) == 4 && pipe_ok<NN>()) {
+        constexpr int WP = lean_w<float, NN, 1>();
+        const unsigned cols = a.axis == 1 ? (unsigned)a.ph.nz : (unsigned)(a.ph.ny * a.ph.nz);
+        if (lean && (a.variant & 4) && a.axis != 2 && a.pipe_wgs > 0 && a.M % WP == 0 && cols % WP == 0) {
+            const unsigned ntiles = a.M / WP;
+            const unsigned pb = ntiles < a.pipe_wgs ? ntiles : a.pipe_wgs;
+#define GPE_PIPE(SH, MKV, CJV, AXV) \
+    hipLaunchKernelGGL((fft_pipe_kernel<NN, SH, MKV, CJV, AXV>), dim3(pb), dim3(kThreads), 0, s, a)
+            if (shape == SHAPE_FI && a.axis == 0) {
+                if (!has_mid) GPE_PIPE(SHAPE_FI, -1, false, 0);
+                else switch (mk) {
+                    case MK_EXPT_HL: if (cj) GPE_PIPE(SHAPE_FI, MK_EXPT_HL, true, 0); else GPE_PIPE(SHAPE_FI, MK_EXPT_HL, false, 0); break;
+                    case MK_EXPT_64: if (cj) GPE_PIPE(SHAPE_FI, MK_EXPT_64, true, 0); else GPE_PIPE(SHAPE_FI, MK_EXPT_64, false, 0); break;
+                    case MK_K2: GPE_PIPE(SHAPE_FI, MK_K2, false, 0); break;
+                    case MK_VPP: GPE_PIPE(SHAPE_FI, MK_VPP, false, 0); break;
+                    default: GPE_PIPE(SHAPE_FI, MK_KA, false, 0); break;
+                }
+                return pb;
+            }
+            if (!has_mid && (shape == SHAPE_F || shape == SHAPE_I)) {
+                if (shape == SHAPE_F) {
+                    if (a.axis == 0) GPE_PIPE(SHAPE_F, -1, false, 0); else GPE_PIPE(SHAPE_F, -1, false, 1);
+                } else {
+                    if (a.axis == 0) GPE_PIPE(SHAPE_I, -1, false, 0); else GPE_PIPE(SHAPE_I, -1, false, 1);
+                }
+                return pb;
+            }
+#undef GPE_PIPE
+        }
+    }
     if (a.axis == 2 && a.ph.modify && !lean) {
         // the physics z passes of engines with user hooks (INTERACTIONS / VEXT 100)
 #define GPE_LAUNCH_UH(SHAPE)                                                                              \
