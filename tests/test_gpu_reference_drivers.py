@@ -93,14 +93,15 @@ RTE_VORTEX = os.path.join(BIN, "rte_vortex_dev")
 @pytest.mark.skipif(not os.path.exists(RTE_VORTEX), reason="reference drivers not built (tests/drivers/ref_drivers.py)")
 def test_reference_rte_vortex_resumes_from_a_dftc_file(tmp_path, oracle):
     """test_vortex/rte_vortex.c on 32^3: it reads <file>.dftc.info with
-    get_info, checks the lattice, loads the frame, evolves 5 blocks of 100
-    steps with an energy call and hands every psi to its writer thread through
+    get_info, checks the lattice, loads the frame, evolves 10 blocks of 100
+    steps (a fixed loop count, rte_vortex.c:322; its snapshot argument is read
+    but unused) with an energy call and hands every psi to its writer thread through
     the frame queue (include/gpe_queue.h, OpenMP sections), which appends it to
     psi_frames.dftc.  Every frame must equal the oracle's replay to 1e-12."""
     from libcugpe_amd import dftc
 
     print("[gpu-case] reference driver rte_vortex from a .dftc file", flush=True)
-    shape, aspect, blocks = (32, 32, 32), 4.0, 5
+    shape, aspect, blocks = (32, 32, 32), 4.0, 10
     params = [0.01, aspect * aspect * 0.01, aspect * aspect * 0.01, 0.0]  # rte_vortex.c:271-275
     o = oracle.OracleEngine(shape, 0.0, 1.0, DT, NPART, 0, 0, 0)
     o.set_user_params(params)
