@@ -404,6 +404,26 @@ void issue_open(gpe_engine_s* e, const StepPlan& sp) {
         run_pass<T>(e, a, {2, nullptr, e->C, -1, 0, LOAD_PSI0, MID_NONE, STORE_PLAIN, imp, 0});
 }
 
+// The kinetic factor of a step on a buffer whose z axis is already in k_z
+// (orginal/gpe_engine.cu:963-966, FFT, exp(c k^2 / gamma^2) / N, inverse FFT).
+// K = K_x K_y K_z factorises by axis and the 1-D transforms along x and y
+// commute with each other and with the factors of the other axes, so the
+// default plan is two turn-around passes: y (forward y, K_y K_z, inverse y)
+// and x (forward x, K_x, inverse x).  Launcher variant bit 3 selects the
+// unsplit plan: forward y, the x turn-around with K_x K_y K_z, inverse y.
+template <typename T>
+void kinetic_kz(gpe_engine_s* e, const PassArgs<T>& a, void* buf) {
+    const int F = -1, I = +1;
+    if (!(e->variant & 8)) {
+        run_pass<T>(e, a, {1, buf, buf, F, I, LOAD_PLAIN, MID_EXPT_SEP, STORE_PLAIN, 0, 0});
+        run_pass<T>(e, a, {0, buf, buf, F, I, LOAD_PLAIN, MID_EXPT_SEP, STORE_PLAIN, 0, 0});
+    } else {
+        run_pass<T>(e, a, {1, buf, buf, F, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+        run_pass<T>(e, a, {0, buf, buf, F, I, LOAD_PLAIN, MID_EXPT, STORE_PLAIN, 0, 0});
+        run_pass<T>(e, a, {1, buf, buf, I, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+    }
+}
+
 // one split step from time index e->it (the host counter is advanced by the caller)
 template <typename T>
 void issue_step(gpe_engine_s* e, const StepPlan& sp, bool last) {
@@ -422,9 +442,7 @@ void issue_step(gpe_engine_s* e, const StepPlan& sp, bool last) {
         run_pass<T>(e, a, {2, C, B, I, F, LOAD_PLAIN, MID_QF_VSTEP1, STORE_PLAIN, F_QF, 0});
     }
     // predictor kinetic step of phi1
-    run_pass<T>(e, a, {1, B, B, F, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
-    run_pass<T>(e, a, {0, B, B, F, I, LOAD_PLAIN, MID_EXPT, STORE_PLAIN, 0, 0});
-    run_pass<T>(e, a, {1, B, B, I, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+    kinetic_kz<T>(e, a, B);
     if (!ite) {
         run_pass<T>(e, a, {2, B, A, I, F, LOAD_PLAIN, MID_VSTEP2, STORE_PLAIN, fqf, 0});
     } else {
@@ -434,9 +452,7 @@ void issue_step(gpe_engine_s* e, const StepPlan& sp, bool last) {
                            F_SCALE | fqf | (sp.mid_pin ? F_IMPRINT_MID : 0), 0});
     }
     // corrector kinetic step of modify(psi0) W
-    run_pass<T>(e, a, {1, A, A, F, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
-    run_pass<T>(e, a, {0, A, A, F, I, LOAD_PLAIN, MID_EXPT, STORE_PLAIN, 0, 0});
-    run_pass<T>(e, a, {1, A, A, I, 0, LOAD_PLAIN, MID_NONE, STORE_PLAIN, 0, 0});
+    kinetic_kz<T>(e, a, A);
     if (!ite) {
         if (!last) {
             const int nxt = (qf ? F_NEXT_PSI : F_NEXT_VSTEP1) | (vortex ? F_IMPRINT_NEXT : 0);
@@ -1612,7 +1628,8 @@ int ens_evolve_t(gpe_ensemble_s* s, int nt) {
 
 // Times `steps` real-time steps of one engine for each launcher variant
 // (bit 0: z passes in LDS tiles instead of wave registers, bit 1: half-width
-// strided tiles) and keeps the fastest.  The state is saved and restored.
+// strided tiles, bit 2: pipelined strided passes) and keeps the fastest; bit 3
+// (the unsplit kinetic plan) is kept as set.  The state is saved and restored.
 template <typename T>
 int autotune_t(gpe_engine_s* e, int steps, int* chosen) {
     if (steps < 1) steps = 1;
@@ -1628,9 +1645,10 @@ int autotune_t(gpe_engine_s* e, int steps, int* chosen) {
     int err = 0;
     (void)hipMemcpyAsync(save, e->A, bytes, hipMemcpyDeviceToDevice, e->stream);
     const double beta = e->beta;
+    const int plan = e->variant & 8;
     for (int v = 0; v < 8 && !err; v++) {
         if ((v & 4) && e->precision != GPE_PRECISION_C64) continue;  // pipelined passes: complex64 only
-        e->variant = v;
+        e->variant = v | plan;
         StepPlan sp;
         // one untimed step, then the timed ones (real-time schedule in both time modes)
         e->beta = 0.0;
@@ -1645,7 +1663,7 @@ int autotune_t(gpe_engine_s* e, int steps, int* chosen) {
         if (h != hipSuccess) { err = (int)h; break; }
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, t0, t1);
-        if (v == 0 || ms < best_ms) { best = v; best_ms = ms; }
+        if (v == 0 || ms < best_ms) { best = v | plan; best_ms = ms; }
     }
     e->beta = beta;
     (void)hipMemcpyAsync(e->A, save, bytes, hipMemcpyDeviceToDevice, e->stream);
@@ -1681,7 +1699,7 @@ int gpe_engine_set_graph(gpe_engine_t e, int on) {
 int gpe_engine_set_variant(gpe_engine_t e, int variant) {
     if (!e) return GPE_ERR_NO_ENGINE;
     e->gen++;
-    if (variant < 0 || variant > 7) return GPE_ERR_ARG;
+    if (variant < 0 || variant > 15) return GPE_ERR_ARG;
     e->variant = variant;
     return 0;
 }

@@ -398,6 +398,9 @@ enum MidMode {
     MID_EXPTC = 10,      // conj(Kx Ky Kz)/N: the adjoint of the kinetic factor
     MID_VPP = 11,        // Vpp(k)/N: dipolar convolution kernel (dev/gpe_engine.cu:960-975)
     MID_VPP_SUM = 12,    // sum Vpp(k) |rho_k|^2 (dev/gpe_engine.cu:1716-1735)
+    // the kinetic factor split by axis, K = (K_y K_z) K_x: K_y K_z in a y
+    // turn-around whose data is already in k_z, K_x in the x turn-around
+    MID_EXPT_SEP = 13,
 };
 
 // Fourier transform of the particle-particle potential of z-polarised dipoles
@@ -574,12 +577,32 @@ __device__ __forceinline__ double dens64(double g, double x, double y) { return 
 
 // k-space factors of the lean passes, one per compile-time kind; MK_EXPT_HL is
 // the complex64 x-axis form of MID_EXPT / MID_EXPTC with the compensated
-// tables, MK_EXPT_64 the FP64 product
-enum MidKind { MK_EXPT_64 = 0, MK_EXPT_HL = 1, MK_K2 = 2, MK_VPP = 3, MK_KA = 4 };
+// tables, MK_EXPT_64 the FP64 product, MK_SEP the per-axis factor of
+// MID_EXPT_SEP (AX: the pass axis, -1 when only known at run time)
+enum MidKind { MK_EXPT_64 = 0, MK_EXPT_HL = 1, MK_K2 = 2, MK_VPP = 3, MK_KA = 4, MK_SEP = 5 };
 
-template <int MK, bool CJ, typename T>
+template <int MK, bool CJ, typename T, int AX = -1>
 __device__ __forceinline__ cplx<T> mid_op_lean_k(const PassArgs<T>& a, const Coord& c, cplx<T> v) {
-    if constexpr (MK == MK_EXPT_HL && sizeof(T) == 4) {
+    if constexpr (MK == MK_SEP) {
+        const bool xpass = (AX >= 0) ? (AX == 0) : (a.axis == 0);
+        if constexpr (sizeof(T) == 4) {
+            // one compensated product per pass with a hi + lo float pair of
+            // an FP64 value: K_x in the x pass, the per-pencil K_y K_z in the
+            // y pass (two roundings per kinetic step, as in the unsplit form)
+            if (xpass) return cmul_comp(v, a.kx_hl[2 * c.ix], a.kx_hl[2 * c.ix + 1]);
+            const unsigned yz = c.iy * (unsigned)a.ph.nz + c.iz;
+            return cmul_comp(v, a.kyz_hl[2 * yz], a.kyz_hl[2 * yz + 1]);
+        } else {
+            cplx<double> w = mk((double)v.x, (double)v.y);
+            if (xpass) {
+                w = cmul(w, a.Kx[c.ix]);
+            } else {
+                w = cmul(w, a.Ky[c.iy]);
+                w = cmul(w, a.Kz[c.iz]);
+            }
+            return mk((T)w.x, (T)w.y);
+        }
+    } else if constexpr (MK == MK_EXPT_HL && sizeof(T) == 4) {
         // complex64 on the x axis: two compensated products with K_x and
         // the per-pencil K_y K_z (hi + lo float pairs of FP64 values)
         const unsigned yz = c.iy * (unsigned)a.ph.nz + c.iz;
@@ -621,6 +644,7 @@ __host__ __device__ __forceinline__ bool mid_kind(const PassArgs<T>& a, int& mk,
     case MID_K2: mk = MK_K2; return true;
     case MID_VPP: mk = MK_VPP; return true;
     case MID_KA: mk = MK_KA; return true;
+    case MID_EXPT_SEP: mk = MK_SEP; return true;
     default: return false;
     }
 }
@@ -635,6 +659,7 @@ __device__ __forceinline__ cplx<T> mid_op_lean(const PassArgs<T>& a, const Coord
     case MK_EXPT_64: return cj ? mid_op_lean_k<MK_EXPT_64, true>(a, c, v) : mid_op_lean_k<MK_EXPT_64, false>(a, c, v);
     case MK_K2: return mid_op_lean_k<MK_K2, false>(a, c, v);
     case MK_VPP: return mid_op_lean_k<MK_VPP, false>(a, c, v);
+    case MK_SEP: return mid_op_lean_k<MK_SEP, false>(a, c, v);
     default: return mid_op_lean_k<MK_KA, false>(a, c, v);
     }
 }
@@ -774,7 +799,8 @@ __device__ __forceinline__ cplx<T> mid_op_m(const PassArgs<T>& a, const ElemIn<T
         const cplx<T> p = e.aux;
         a.rout[off] = (double)(p.x * v.x + p.y * v.y);
         return v;
-    } else if constexpr (MM == MID_EXPT || MM == MID_EXPTC || MM == MID_K2 || MM == MID_KA || MM == MID_VPP) {
+    } else if constexpr (MM == MID_EXPT || MM == MID_EXPTC || MM == MID_K2 || MM == MID_KA || MM == MID_VPP ||
+                         MM == MID_EXPT_SEP) {
         return mid_op_lean(a, e.c, v);
     } else {
         return v;
@@ -795,7 +821,8 @@ __device__ __forceinline__ cplx<T> mid_op2(const PassArgs<T>& a, const ElemIn<T>
     case MID_EXPTC:
     case MID_K2:
     case MID_KA:
-    case MID_VPP: return mid_op_lean(a, e.c, v);
+    case MID_VPP:
+    case MID_EXPT_SEP: return mid_op_lean(a, e.c, v);
     default: return v;
     }
 }
@@ -1024,7 +1051,7 @@ __device__ __forceinline__ void mid_regs_k(const PassArgs<T>& a, unsigned p0, cp
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const Coord c = elem<AX>(a, pc, (unsigned)(j + r * M));
-                v[q][r] = mid_op_lean_k<MK, CJ>(a, c, v[q][r]);
+                v[q][r] = mid_op_lean_k<MK, CJ, T, AX>(a, c, v[q][r]);
             }
         }
     }
@@ -1449,7 +1476,7 @@ template <typename T>
 inline bool pass_is_lean(const PassArgs<T>& a) {
     return a.load_mode == LOAD_PLAIN && a.flags == 0 && a.nred == 0 &&
            (a.mid_mode == MID_NONE || a.mid_mode == MID_EXPT || a.mid_mode == MID_EXPTC || a.mid_mode == MID_K2 ||
-            a.mid_mode == MID_KA || a.mid_mode == MID_VPP);
+            a.mid_mode == MID_KA || a.mid_mode == MID_VPP || a.mid_mode == MID_EXPT_SEP);
 }
 
 inline int pass_shape(int dir1, int dir2) {

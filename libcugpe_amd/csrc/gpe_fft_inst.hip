@@ -35,15 +35,25 @@ unsigned launch_fft_pass_n_impl(const PassArgs<T>& a, hipStream_t s) {
             hipLaunchKernelGGL((fft_pass_kernel<T, NN, false, false, SHAPE, WD, MK, CJ, 1>), dim3(blocks),     \
                                dim3(kThreads), 0, s, a);                                                       \
     } while (0)
-// the turn-around with a mid op is always the x pass
+// turn-arounds with a mid op: the x pass, and the y pass of the split kinetic factor
 #define GPE_LAUNCH_MID(WD, MK, CJ)                                                                            \
     hipLaunchKernelGGL((fft_pass_kernel<T, NN, false, false, SHAPE_FI, WD, MK, CJ, 0>), dim3(blocks),          \
                        dim3(kThreads), 0, s, a)
+#define GPE_LAUNCH_SEP(WD)                                                                                     \
+    do {                                                                                                       \
+        if (a.axis == 0)                                                                                       \
+            hipLaunchKernelGGL((fft_pass_kernel<T, NN, false, false, SHAPE_FI, WD, MK_SEP, false, 0>),         \
+                               dim3(blocks), dim3(kThreads), 0, s, a);                                         \
+        else                                                                                                   \
+            hipLaunchKernelGGL((fft_pass_kernel<T, NN, false, false, SHAPE_FI, WD, MK_SEP, false, 1>),         \
+                               dim3(blocks), dim3(kThreads), 0, s, a);                                         \
+    } while (0)
     int mk = -1;
     bool cj = false;
     const bool has_mid = lean && mid_kind(a, mk, cj);
-    // lean mid ops live in the turn-around of the x axis
-    if (lean && has_mid && (shape != SHAPE_FI || a.axis != 0)) return 0;
+    // lean mid ops live in turn-arounds of the x axis, and of the y axis for
+    // the split kinetic factor
+    if (lean && has_mid && (shape != SHAPE_FI || a.axis == 2 || (a.axis == 1 && mk != MK_SEP))) return 0;
     // pipelined persistent strided passes (variant bit 2, complex64): every
     // tile must be W whole rows of W contiguous columns inside one lattice
     if constexpr (sizeof(T) == 4 && pipe_ok<NN>()) {
@@ -54,6 +64,10 @@ unsigned launch_fft_pass_n_impl(const PassArgs<T>& a, hipStream_t s) {
             const unsigned pb = ntiles < a.pipe_wgs ? ntiles : a.pipe_wgs;
 #define GPE_PIPE(SH, MKV, CJV, AXV) \
     hipLaunchKernelGGL((fft_pipe_kernel<NN, SH, MKV, CJV, AXV>), dim3(pb), dim3(kThreads), 0, s, a)
+            if (shape == SHAPE_FI && has_mid && mk == MK_SEP) {
+                if (a.axis == 0) GPE_PIPE(SHAPE_FI, MK_SEP, false, 0); else GPE_PIPE(SHAPE_FI, MK_SEP, false, 1);
+                return pb;
+            }
             if (shape == SHAPE_FI && a.axis == 0) {
                 if (!has_mid) GPE_PIPE(SHAPE_FI, -1, false, 0);
                 else switch (mk) {
@@ -97,6 +111,9 @@ unsigned launch_fft_pass_n_impl(const PassArgs<T>& a, hipStream_t s) {
         if (!has_mid) {
             if (half) GPE_LAUNCH_S(SHAPE_FI, 2, -1, false);
             else GPE_LAUNCH_S(SHAPE_FI, 1, -1, false);
+        } else if (mk == MK_SEP) {
+            if (half) GPE_LAUNCH_SEP(2);
+            else GPE_LAUNCH_SEP(1);
         } else if (half) {
             switch (mk) {
             case MK_EXPT_HL: if (cj) GPE_LAUNCH_MID(2, MK_EXPT_HL, true); else GPE_LAUNCH_MID(2, MK_EXPT_HL, false); break;
@@ -129,6 +146,7 @@ unsigned launch_fft_pass_n_impl(const PassArgs<T>& a, hipStream_t s) {
         else return 0;
     }
 #undef GPE_LAUNCH_S
+#undef GPE_LAUNCH_SEP
 #undef GPE_LAUNCH_MID
 #undef GPE_LAUNCH
     return blocks;

@@ -299,11 +299,13 @@ def test_pinned_variants_match_oracle(oracle, variant, mode):
     g.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 8, 11])
 @pytest.mark.parametrize("shape", [(32, 32, 32), (48, 8, 64), (16, 96, 8), (8, 12, 48), (16, 8, 24)])
 def test_launcher_variants_match_oracle(oracle, variant, shape):
     """Every launcher choice of the autotuner (z passes in wave registers or LDS
-    tiles, full or half-width strided tiles) follows the oracle's step."""
+    tiles, full or half-width strided tiles) follows the oracle's step, with
+    the kinetic factor split over the y and x turn-arounds (the default) and
+    unsplit in the x turn-around (bit 3)."""
     say("variant", variant, shape)
     for mode in ("rte", "ite"):
         alpha, beta = (1.0, 0.0) if mode == "rte" else (0.0, 1.0)
