@@ -59,6 +59,7 @@ typedef double _Complex cplx; /* C99 complex128, dev/gpe_complex.cuh:36 */
 #define GPE_ERR_ARG (-12)            /* invalid argument                              */
 #define GPE_ERR_VORTEX_NOT_SET (-13) /* gpe_evolve_vortex before gpe_set_vortex       */
 #define GPE_ERR_PHASE_NOT_SET (-15)  /* enforced-phase evolution without a phase      */
+#define GPE_ERR_TOO_LARGE (-16)      /* lattice above gpe_detect_vortices_noalloc's limit */
 #define GPE_ERR_GAMMA (-99)          /* GPE_FOR not PARTICLES / DIMERS (reference code) */
 
 #define GPE_PRECISION_C128 128
@@ -231,6 +232,22 @@ int gpe_engine_set_variant(gpe_engine_t e, int variant);
  * launch every pass directly.  The results are the same bit for bit (no
  * counterpart in the reference). */
 int gpe_engine_set_graph(gpe_engine_t e, int on);
+
+/* Vortex detection that allocates nothing: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The windings and
+ * count of gpe_engine_detect_vortices, computed by the same kernels from a
+ * wave function in the caller's device memory: d_psi holds nx*ny*nz values in
+ * C order (complex64 for precision GPE_PRECISION_C64, complex128 for
+ * GPE_PRECISION_C128); d_windings (device memory, 3*nx*ny*nz int8) receives
+ * windings[3][N]; *d_count (device memory, NULL allowed) receives the number
+ * of non-zero windings as a double.  The work is enqueued on `stream` (a
+ * hipStream_t, NULL for the null stream) and not waited for.  The reduction's
+ * partial sums live in a fixed 8.2 KiB device array of the library, one per
+ * GPU, which every call on that GPU shares: calls that overlap on one GPU
+ * race.  nx*ny*nz above 2^31 - 1 returns GPE_ERR_TOO_LARGE, before any launch. */
+int gpe_detect_vortices_noalloc(const void *d_psi, int nx, int ny, int nz, int precision, double rho_min,
+                                int8_t *d_windings, double *d_count, void *stream);
 
 /* Adjoint of a real-time run (no counterpart in the reference).  forward:
  * nt steps of gpe_evolve_qf from the current psi, keeping every ckpt-th state

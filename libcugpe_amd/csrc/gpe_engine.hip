@@ -1137,6 +1137,7 @@ const char* gpe_get_error_string(gpe_result_t res) {
     case GPE_ERR_ARG: return "invalid argument";
     case GPE_ERR_VORTEX_NOT_SET: return "vortex not set (gpe_set_vortex)";
     case GPE_ERR_PHASE_NOT_SET: return "phase not set (gpe_set_phase / gpe_get_phase)";
+    case GPE_ERR_TOO_LARGE: return "lattice too large for gpe_detect_vortices_noalloc (nx*ny*nz < 2^31)";
     case GPE_ERR_GAMMA: return "GPE_FOR must be PARTICLES or DIMERS";
     default: break;
     }

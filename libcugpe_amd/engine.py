@@ -72,6 +72,7 @@ def lib():
             "gpe_engine_get_phase": [_p, _p],
             "gpe_engine_set_phase": [_p, _p],
             "gpe_engine_detect_vortices": [_p, _d, _p, ctypes.POINTER(ctypes.c_long)],
+            "gpe_detect_vortices_noalloc": [_p, _i, _i, _i, _i, _d, _p, _p, _p],
             "gpe_engine_get_time": [_p, _dp],
             "gpe_engine_fft": [_p, _i],
             "gpe_engine_get_config": [_p, ctypes.POINTER(Config)],
