@@ -222,8 +222,13 @@ const char *gpe_build_info(void);
  * LDS tiles instead of wave registers, bit 1 halves the tile width of the
  * strided passes, bit 2 runs the strided passes of complex64 engines in
  * persistent workgroups that copy the next tile into LDS while transforming
- * the current one.  autotune times `steps` real-time steps per choice on the
- * current state (restored afterwards) and keeps the fastest. */
+ * the current one, bit 3 runs the kinetic factor in one x turn-around pass
+ * between forward and inverse y passes, and bit 4 runs whole real-time steps
+ * of cubes that fit in one workgroup's LDS (complex64 8^3 to 24^3,
+ * complex128 8^3 to 16^3; no quantum friction, no pinning) in one kernel per
+ * lattice, the lower bits serving the runs it does not cover.  autotune times
+ * `steps` real-time steps per choice on the current state (restored
+ * afterwards) and keeps the fastest. */
 int gpe_engine_autotune(gpe_engine_t e, int steps, int *variant);
 int gpe_engine_set_variant(gpe_engine_t e, int variant);
 /* 1: an evolve call of 3 or more steps replays HIP graphs of the open pass,
@@ -295,6 +300,8 @@ int gpe_ensemble_evolve(gpe_ensemble_t s, int nt);
 /* 1: replay captured HIP graphs (default), 0: launch the passes directly */
 int gpe_ensemble_set_graph(gpe_ensemble_t s, int on);
 int gpe_ensemble_autotune(gpe_ensemble_t s, int steps);
+/* launcher variant of one group (the bits of gpe_engine_set_variant) */
+int gpe_ensemble_set_variant(gpe_ensemble_t s, int group, int variant);
 int gpe_ensemble_groups(gpe_ensemble_t s, int *ngroups);
 int gpe_ensemble_group_info(gpe_ensemble_t s, int group, int *nx, int *ny, int *nz, int *count, int *variant);
 int gpe_ensemble_instance(gpe_ensemble_t s, int inst, int *group, int *slot);

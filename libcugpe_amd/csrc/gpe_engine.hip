@@ -477,9 +477,56 @@ void issue_step(gpe_engine_s* e, const StepPlan& sp, bool last) {
     if (e->uses_hooks) launch_tick(e->itdev, e->stream);
 }
 
+// Launcher variant bit 4: whole real-time steps in one kernel per lattice
+// (fused_steps_kernel), for cubes whose state fits in one workgroup's LDS.
+// The bits below it still choose the passes of the runs the fused kernel does
+// not cover (imaginary or complex time, quantum friction, pinning).
+constexpr int kVariantFused = 16;
+
+template <typename T>
+bool fused_eligible(const gpe_engine_s* e, int pin) {
+    return e->nx == e->ny && e->ny == e->nz && fused_length_ok<T>(e->nx) && e->beta == 0.0 && e->qf == 0.0 &&
+           pin == 0 && e->interactions != INT_DIPOLAR;
+}
+
+template <typename T>
+bool fused_selected(const gpe_engine_s* e, int pin) {
+    return (e->variant & kVariantFused) && e->prof_class < 0 && fused_eligible<T>(e, pin);
+}
+
+// nt real-time steps of every lattice of the engine from time index e->it
+// (the host counter is advanced by the caller), in launches of at most 256
+// steps so that no single kernel runs for long
+template <typename T>
+int fused_run(gpe_engine_s* e, int nt, hipStream_t s) {
+    const PassArgs<T> a = base_args<T>(e);
+    FusedArgs<T> f;
+    std::memset(&f, 0, sizeof(f));
+    f.psi = (cplx<T>*)e->A;
+    f.Kx = a.Kx;
+    f.Ky = a.Ky;
+    f.Kz = a.Kz;
+    f.kx_hl = a.kx_hl;
+    f.kyz_hl = a.kyz_hl;
+    f.ph = a.ph;
+    constexpr int kChunk = 256;
+    for (int done = 0; done < nt; done += kChunk) {
+        f.nt = std::min(kChunk, nt - done);
+        f.it0 = e->it + (unsigned)done;
+        if (launch_fused_steps<T>(f, e->nx, (unsigned)e->batch, e->uses_hooks, s)) return GPE_ERR_ARG;
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 template <typename T>
 int evolve_t(gpe_engine_s* e, int nt, double* mu, int pin, bool mid_pin) {
     if (nt <= 0) return 0;
+    if (fused_selected<T>(e, pin)) {
+        if (int r = fused_run<T>(e, nt, e->stream)) return r;
+        e->it = e->it + (unsigned)nt;
+        return 0;
+    }
     const bool ite = (e->beta != 0.0);
     // lattices of a batch share one stream of passes: imaginary time (one
     // norm per lattice) and a stored phase (one array) are single-lattice only
@@ -1574,6 +1621,7 @@ void ens_issue(gpe_ensemble_s* s, int what) {
     StepPlan sp;
     for (size_t i = 0; i < s->groups.size(); i++) {
         gpe_engine_s* g = s->groups[i];
+        if (fused_selected<T>(g, 0)) continue;  // whole steps in one kernel, launched by ens_evolve_t
         (void)hipStreamWaitEvent(g->stream, s->fork, 0);
         if (what == 0) issue_open<T>(g, sp);
         else issue_step<T>(g, sp, what == 2);
@@ -1608,6 +1656,16 @@ int ens_evolve_t(gpe_ensemble_s* s, int nt) {
     for (auto* g : s->groups) HIP_TRY(hipStreamSynchronize(g->stream));
     for (auto* g : s->groups)
         if (g->uses_hooks) launch_set_counter(g->itdev, g->it, s->main);
+    // groups on the fused whole-step kernel: nt steps on the group's stream,
+    // beside the passes of the other groups
+    (void)hipEventRecord(s->fork, s->main);
+    for (size_t i = 0; i < s->groups.size(); i++) {
+        gpe_engine_s* g = s->groups[i];
+        if (!fused_selected<T>(g, 0)) continue;
+        HIP_TRY(hipStreamWaitEvent(g->stream, s->fork, 0));
+        if (int r = fused_run<T>(g, nt, g->stream)) return r;
+        HIP_TRY(hipEventRecord(s->join[i], g->stream));
+    }
     if (s->use_graph) {
         if (!s->graphs_ok) {
             int r = ens_build_graphs<T>(s);
@@ -1621,6 +1679,8 @@ int ens_evolve_t(gpe_ensemble_s* s, int nt) {
         for (int i = 0; i < nt - 1; i++) ens_issue<T>(s, 1);
         ens_issue<T>(s, 2);
     }
+    for (size_t i = 0; i < s->groups.size(); i++)
+        if (fused_selected<T>(s->groups[i], 0)) HIP_TRY(hipStreamWaitEvent(s->main, s->join[i], 0));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->main));
     for (auto* g : s->groups) g->it += (unsigned)nt;
@@ -1630,7 +1690,9 @@ int ens_evolve_t(gpe_ensemble_s* s, int nt) {
 // Times `steps` real-time steps of one engine for each launcher variant
 // (bit 0: z passes in LDS tiles instead of wave registers, bit 1: half-width
 // strided tiles, bit 2: pipelined strided passes) and keeps the fastest; bit 3
-// (the unsplit kinetic plan) is kept as set.  The state is saved and restored.
+// (the unsplit kinetic plan) is kept as set.  Where the fused whole-step kernel
+// applies, bit 4 is added when it beats the fastest passes.  The state is saved
+// and restored.
 template <typename T>
 int autotune_t(gpe_engine_s* e, int steps, int* chosen) {
     if (steps < 1) steps = 1;
@@ -1667,6 +1729,21 @@ int autotune_t(gpe_engine_s* e, int steps, int* chosen) {
         if (v == 0 || ms < best_ms) { best = v | plan; best_ms = ms; }
     }
     e->beta = beta;
+    // the fused whole-step kernel where it applies, keeping the best passes
+    // for the runs it does not cover
+    if (!err && fused_eligible<T>(e, 0)) {
+        e->variant = best | kVariantFused;
+        err = fused_run<T>(e, 1, e->stream);
+        (void)hipEventRecord(t0, e->stream);
+        if (!err) err = fused_run<T>(e, steps, e->stream);
+        (void)hipEventRecord(t1, e->stream);
+        if (!err) {
+            const hipError_t h = hipEventSynchronize(t1);
+            float ms = 0.f;
+            if (h != hipSuccess) err = (int)h;
+            else if (hipEventElapsedTime(&ms, t0, t1) == hipSuccess && ms < best_ms) best |= kVariantFused;
+        }
+    }
     (void)hipMemcpyAsync(e->A, save, bytes, hipMemcpyDeviceToDevice, e->stream);
     (void)hipStreamSynchronize(e->stream);
     (void)hipFree(save);
@@ -1700,7 +1777,7 @@ int gpe_engine_set_graph(gpe_engine_t e, int on) {
 int gpe_engine_set_variant(gpe_engine_t e, int variant) {
     if (!e) return GPE_ERR_NO_ENGINE;
     e->gen++;
-    if (variant < 0 || variant > 15) return GPE_ERR_ARG;
+    if (variant < 0 || variant > 31) return GPE_ERR_ARG;
     e->variant = variant;
     return 0;
 }
@@ -1845,6 +1922,13 @@ int gpe_ensemble_autotune(gpe_ensemble_t s, int steps) {
         if (r) return r;
     }
     return 0;
+}
+
+int gpe_ensemble_set_variant(gpe_ensemble_t s, int group, int variant) {
+    if (!s) return GPE_ERR_NO_ENGINE;
+    if (group < 0 || group >= (int)s->groups.size()) return GPE_ERR_ARG;
+    ens_drop_graphs(s);
+    return gpe_engine_set_variant(s->groups[group], variant);
 }
 
 int gpe_ensemble_groups(gpe_ensemble_t s, int* ngroups) {

@@ -136,7 +136,10 @@ struct Roots32Odd {
                                      -0.19509032201612826785};
 };
 
-// a * exp(S 2 pi i E / R) for R dividing 32 or 12.  complex64 takes a
+// exact cos and sin of 2 pi / 24 (15 degrees): (sqrt 6 + sqrt 2) / 4 and (sqrt 6 - sqrt 2) / 4
+constexpr double kCos15 = 0.96592582628906828675, kSin15 = 0.25881904510252076235;
+
+// a * exp(S 2 pi i E / R) for R dividing 32, 24 or 12.  complex64 takes a
 // constant as a float hi + lo pair, so that its rounding does not shrink |a|
 // the same way on every application
 template <int R, int E, int S, typename T>
@@ -154,6 +157,30 @@ __device__ __forceinline__ cplx<T> rot(cplx<T> a) {
         } else {
             constexpr double cv = Roots12::c[j];
             constexpr double sv = S * Roots12::s[j];
+            if constexpr (sizeof(T) == 4) {
+                constexpr float ch = (float)cv, cl = (float)(cv - (double)ch);
+                constexpr float sh = (float)sv, sl = (float)(sv - (double)sh);
+                return mk(fmaf(a.x, ch, fmaf(-a.y, sh, a.x * cl - a.y * sl)),
+                          fmaf(a.x, sh, fmaf(a.y, ch, a.x * sl + a.y * cl)));
+            } else {
+                const T c = (T)cv;
+                const T s = (T)sv;
+                return mk(a.x * c - a.y * s, a.x * s + a.y * c);
+            }
+        }
+    } else if constexpr (R == 24) {
+        constexpr int e = E % 24;
+        if constexpr (e % 2 == 0) {
+            return rot<12, e / 2, S>(a);
+        } else if constexpr (e % 3 == 0) {
+            return rot<8, e / 3, S>(a);
+        } else {
+            // odd multiples of 15 degrees that are not multiples of 45
+            constexpr double cv = (e == 1 || e == 23) ? kCos15 : (e == 5 || e == 19) ? kSin15
+                                  : (e == 7 || e == 17) ? -kSin15 : -kCos15;
+            constexpr double s0 = (e == 1 || e == 11) ? kSin15 : (e == 5 || e == 7) ? kCos15
+                                  : (e == 13 || e == 23) ? -kSin15 : -kCos15;
+            constexpr double sv = S * s0;
             if constexpr (sizeof(T) == 4) {
                 constexpr float ch = (float)cv, cl = (float)(cv - (double)ch);
                 constexpr float sh = (float)sv, sl = (float)(sv - (double)sh);
@@ -322,6 +349,7 @@ __device__ __forceinline__ void dft(cplx<T>* v) {
     else if constexpr (R == 8) dft_composite<4, 2, S>(v);
     else if constexpr (R == 12) dft_composite<4, 3, S>(v);
     else if constexpr (R == 16) dft_composite<4, 4, S>(v);
+    else if constexpr (R == 24) dft_composite<8, 3, S>(v);
     else if constexpr (R == 32) dft_composite<8, 4, S>(v);
     else static_assert(R == 2, "unsupported radix");
 }

@@ -4,6 +4,7 @@
 
 #include "gpe_fft.hpp"
 #include "gpe_zwave.hpp"
+#include "gpe_fused.hpp"
 
 namespace gpe {
 
@@ -87,6 +88,15 @@ void launch_tick(unsigned* c, hipStream_t s);
 void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 constexpr int kMaxReduceBlocks = 1024;
+
+// whole real-time steps of L^3 lattices held in one workgroup's LDS
+// (gpe_fused.hip): the cube lengths compiled per precision, and nt steps of
+// `batch` lattices stored one after another (uh: the user hook kernels);
+// returns non-zero for a length without a kernel
+template <typename T>
+bool fused_length_ok(int L);
+template <typename T>
+int launch_fused_steps(const FusedArgs<T>& a, int L, unsigned batch, bool uh, hipStream_t s);
 
 // dipolar path (gpe_dipolar.hip)
 template <typename T>

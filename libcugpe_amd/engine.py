@@ -100,6 +100,7 @@ def lib():
             "gpe_ensemble_evolve": [_p, _i],
             "gpe_ensemble_set_graph": [_p, _i],
             "gpe_ensemble_autotune": [_p, _i],
+            "gpe_ensemble_set_variant": [_p, _i, _i],
             "gpe_ensemble_groups": [_p, ctypes.POINTER(_i)],
             "gpe_ensemble_group_info": [_p, _i] + [ctypes.POINTER(_i)] * 5,
             "gpe_ensemble_instance": [_p, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)],
@@ -412,6 +413,10 @@ class Ensemble:
 
     def autotune(self, steps=3):
         _check(lib().gpe_ensemble_autotune(self.h, int(steps)), "gpe_ensemble_autotune")
+
+    def set_variant(self, group, variant):
+        """Launcher variant of one group (bits of Engine.set_variant; 16: fused whole steps)."""
+        _check(lib().gpe_ensemble_set_variant(self.h, int(group), int(variant)), "gpe_ensemble_set_variant")
 
     def groups(self):
         """[(shape, count, variant)] per group."""
