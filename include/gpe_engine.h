@@ -190,6 +190,12 @@ int gpe_engine_get_currents(gpe_engine_t e, double *t, double *jx, double *jy, d
 int gpe_engine_get_phase(gpe_engine_t e, double *h_phase);
 int gpe_engine_set_phase(gpe_engine_t e, const double *h_phase);
 int gpe_engine_detect_vortices(gpe_engine_t e, double rho_min, int8_t *windings, long *count);
+/* Cloud parameters computed on the device (the host-array forms are in
+ * gpe_cloud_params.h, dev/gpe_cloud_params.h:60-119): out[0] density at the
+ * lattice centre, out[1] largest density, out[2..4] cloud range along x, y, z
+ * (first distance from the centre where the density is below epsilon, NaN
+ * when none is). */
+int gpe_engine_cloud_params(gpe_engine_t e, double epsilon, double *out);
 int gpe_engine_evolve_dipolar(gpe_engine_t e, int nt, double *chemical_potential);
 /* method 1: gpe_energy_dipolar, 2: gpe_energy_dipolar2 */
 int gpe_engine_energy_dipolar(gpe_engine_t e, int method, double *t, double *edip);

@@ -815,6 +815,39 @@ int ready_to_launch(const gpe_engine_s* e) {
     return 0;
 }
 
+// Cloud parameters of the state (dev/gpe_cloud_params.h:60-119) from one
+// kernel over the device array instead of a host copy of psi:
+//   out[0] the density at (nx/2, ny/2, nz/2)                        find_n0
+//   out[1] the largest density                                      find_max_density
+//   out[2..4] along the lines through the centre, the first distance
+//   nx/2 - ix (ny/2 - iy, nz/2 - iz) with density below epsilon,
+//   scanning ix = nx/2 down to 1; NaN when no point there is below  cloud_range_ix/iy/iz
+template <typename T>
+int cloud_t(gpe_engine_s* e, double epsilon, double* out) {
+    if (int r = ensure_rout(e)) return r;
+    const int nx = e->nx, ny = e->ny, nz = e->nz;
+    const int nb = launch_cloud<T>((const cplx<T>*)e->A, nx, ny, nz, e->gamma, e->partials, e->rout, e->stream);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> pm((size_t)nb), ln((size_t)(nx + ny + nz));
+    HIP_TRY(hipMemcpyAsync(pm.data(), e->partials, pm.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(ln.data(), e->rout, ln.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    double m = 0.0;
+    for (double v : pm) m = std::max(m, v);
+    out[0] = ln[(size_t)(nx / 2)];
+    out[1] = m;
+    const int len[3] = {nx, ny, nz}, base[3] = {0, nx, nx + ny};
+    for (int a = 0; a < 3; a++) {
+        out[2 + a] = std::nan("");
+        for (int i = len[a] / 2; i > 0; i--)
+            if (ln[(size_t)(base[a] + i)] < epsilon) {
+                out[2 + a] = (double)(len[a] / 2 - i);
+                break;
+            }
+    }
+    return 0;
+}
+
 template <typename T>
 int detect_t(gpe_engine_s* e, double rho_min, int8_t* w, long* count) {
     if (!e->wind) HIP_TRY(hipMalloc(&e->wind, 3 * e->n));
@@ -1067,6 +1100,13 @@ int gpe_engine_set_phase(gpe_engine_t e, const double* h_phase) {
 int gpe_engine_detect_vortices(gpe_engine_t e, double rho_min, int8_t* windings, long* count) {
     if (!e) return GPE_ERR_NO_ENGINE;
     return DISPATCH(e, detect_t, e, rho_min, windings, count);
+}
+
+int gpe_engine_cloud_params(gpe_engine_t e, double epsilon, double* out) {
+    if (!e) return GPE_ERR_NO_ENGINE;
+    if (!out) return GPE_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return DISPATCH(e, cloud_t, e, epsilon, out);
 }
 
 int gpe_engine_fft(gpe_engine_t e, int sign) {

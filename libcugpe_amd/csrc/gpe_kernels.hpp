@@ -68,6 +68,13 @@ template <typename T>
 int launch_windings(const cplx<T>* A, int nx, int ny, int nz, double rho_min, int8_t* w, double* partials,
                     hipStream_t s);
 
+// per-workgroup maxima of gamma |psi|^2 into partials[0..nb) (nb returned, at
+// most kMaxReduceBlocks) and the densities along the three lines through the
+// lattice centre into lines[nx | ny | nz] (FP64)
+template <typename T>
+int launch_cloud(const cplx<T>* A, int nx, int ny, int nz, double gamma, double* partials, double* lines,
+                 hipStream_t s);
+
 // phase (arg psi) into a FP64 array, and enforcing a stored phase: psi = |psi| exp(i phase)
 template <typename T>
 void launch_phase(const cplx<T>* A, unsigned n, double* out, hipStream_t s);

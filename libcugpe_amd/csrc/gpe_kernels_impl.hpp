@@ -213,6 +213,43 @@ static inline int reduce_grid(unsigned n) {
     return (int)b;
 }
 
+// Cloud parameters (dev/gpe_cloud_params.h:60-119) on the device: the
+// largest gamma |psi|^2 of each workgroup into partials[blockIdx], and from
+// workgroup 0 the densities along the three lines through the lattice centre
+// (nx/2, ny/2, nz/2) into lines[nx | ny | nz].
+template <typename T>
+__global__ __launch_bounds__(kThreads) void cloud_kernel(const cplx<T>* __restrict__ A, unsigned n, int nx, int ny,
+                                                         int nz, double gamma, double* __restrict__ partials,
+                                                         double* __restrict__ lines) {
+    __shared__ double red[kThreads / 64];
+    double m = 0.0;
+    const unsigned stride = gridDim.x * kThreads;
+    for (unsigned i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+        const cplx<T> v = A[i];
+        m = fmax(m, gamma * ((double)v.x * (double)v.x + (double)v.y * (double)v.y));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kThreads / 64; w++) m = fmax(m, red[w]);
+        partials[blockIdx.x] = m;
+    }
+    if (blockIdx.x == 0) {
+        const unsigned cx = (unsigned)(nx / 2), cy = (unsigned)(ny / 2), cz = (unsigned)(nz / 2);
+        const unsigned nl = (unsigned)(nx + ny + nz);
+        for (unsigned i = threadIdx.x; i < nl; i += kThreads) {
+            unsigned ix = cx, iy = cy, iz = cz;
+            if (i < (unsigned)nx) ix = i;
+            else if (i < (unsigned)(nx + ny)) iy = i - (unsigned)nx;
+            else iz = i - (unsigned)(nx + ny);
+            const cplx<T> v = A[((size_t)ix * (unsigned)ny + iy) * (unsigned)nz + iz];
+            lines[i] = gamma * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+        }
+    }
+}
+
 template <typename T>
 int launch_norm_partials_impl(const cplx<T>* A, unsigned n, double gamma, double* partials, hipStream_t s) {
     const int nb = reduce_grid(n);
@@ -264,6 +301,16 @@ static inline dim3 ew_grid(unsigned n) { return dim3((n + kThreads - 1) / kThrea
     int launch_windings<T>(const cplx<T>* A, int nx, int ny, int nz, double rho_min, int8_t* w, double* partials, \
                            hipStream_t s) {                                                                       \
         return launch_windings_impl<T>(A, nx, ny, nz, rho_min, w, partials, s);                                   \
+    }                                                                                                             \
+    template <>                                                                                                   \
+    int launch_cloud<T>(const cplx<T>* A, int nx, int ny, int nz, double gamma, double* partials, double* lines,  \
+                        hipStream_t s) {                                                                          \
+        const unsigned n = (unsigned)nx * (unsigned)ny * (unsigned)nz;                                            \
+        unsigned nb = (n + kThreads - 1) / kThreads;                                                              \
+        if (nb > (unsigned)kMaxReduceBlocks) nb = kMaxReduceBlocks;                                               \
+        hipLaunchKernelGGL((cloud_kernel<T>), dim3(nb), dim3(kThreads), 0, s, A, n, nx, ny, nz, gamma, partials,  \
+                           lines);                                                                                \
+        return (int)nb;                                                                                           \
     }                                                                                                             \
     template <>                                                                                                   \
     void launch_phase<T>(const cplx<T>* A, unsigned n, double* out, hipStream_t s) {                              \

@@ -72,6 +72,7 @@ def lib():
             "gpe_engine_get_phase": [_p, _p],
             "gpe_engine_set_phase": [_p, _p],
             "gpe_engine_detect_vortices": [_p, _d, _p, ctypes.POINTER(ctypes.c_long)],
+            "gpe_engine_cloud_params": [_p, _d, _p],
             "gpe_detect_vortices_noalloc": [_p, _i, _i, _i, _i, _d, _p, _p, _p],
             "gpe_engine_get_time": [_p, _dp],
             "gpe_engine_fft": [_p, _i],
@@ -269,6 +270,14 @@ class Engine:
         _check(lib().gpe_engine_detect_vortices(self._h, rho_min, _ptr(w) if w is not None else None,
                                                 ctypes.byref(cnt)), "gpe_detect_vortices")
         return w, cnt.value
+
+    def cloud_params(self, epsilon=1e-8):
+        """(n0, max density, range x, range y, range z) computed on the device
+        (dev/gpe_cloud_params.h:60-119: density at the centre, largest density,
+        first distance from the centre with density below epsilon, NaN if none)."""
+        out = np.empty(5, dtype=np.float64)
+        _check(lib().gpe_engine_cloud_params(self._h, float(epsilon), _ptr(out)), "gpe_engine_cloud_params")
+        return tuple(float(x) for x in out)
 
     def fft(self, sign):
         """In-place unnormalised 3-D DFT of psi (sign -1 forward)."""
