@@ -1615,8 +1615,10 @@ int gpe_engine_adjoint_backward(gpe_engine_t e, double* grad_vext) {
 // forked from and joined back into the ensemble stream, and is captured once
 // as a HIP graph: open, middle and last step graphs, replayed nt - 2 times for
 // the middle.  Kernel arguments are frozen in the graphs, so the graphs are
-// rebuilt after every parameter change; the time index is one of those
-// arguments and is read by no built-in potential (all are time independent).
+// rebuilt after every parameter change; the user hooks read the time index
+// from a device counter advanced after every step, not from the arguments.
+// Groups on the fused whole-step kernel run outside the graphs, on their own
+// streams, for all nt steps of a call.
 
 struct gpe_ensemble_s {
     std::vector<gpe_engine_s*> groups;
