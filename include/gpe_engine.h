@@ -232,7 +232,10 @@ const char *gpe_build_info(void);
  * between forward and inverse y passes, and bit 4 runs whole real-time steps
  * of cubes that fit in one workgroup's LDS (complex64 8^3 to 24^3,
  * complex128 8^3 to 16^3; no quantum friction, no pinning) in one kernel per
- * lattice, the lower bits serving the runs it does not cover.  autotune times
+ * lattice, the lower bits serving the runs it does not cover, and bit 5
+ * runs y-z planes of ny = nz = 32, 48 or 64 points in one pass (z and y
+ * transforms with the physics between them: four passes per real-time step
+ * instead of six; no quantum friction, no pinning).  autotune times
  * `steps` real-time steps per choice on the current state (restored
  * afterwards) and keeps the fastest. */
 int gpe_engine_autotune(gpe_engine_t e, int steps, int *variant);

@@ -73,7 +73,7 @@ def jobs():
         for n in fft_lengths():
             obj = os.path.join(BUILD, "fft_%s_%d.o" % (real, n))
             out.append((obj, [inst], ["-DGPE_REAL=" + real, "-DGPE_FFT_N=%d" % n], hdrs))
-    for name in ("gpe_kernels_f32", "gpe_kernels_f64", "gpe_adjoint", "gpe_dipolar", "gpe_engine", "gpe_noalloc", "gpe_fused"):
+    for name in ("gpe_kernels_f32", "gpe_kernels_f64", "gpe_adjoint", "gpe_dipolar", "gpe_engine", "gpe_noalloc", "gpe_fused", "gpe_plane"):
         src = os.path.join(CSRC, name + ".hip")
         out.append((os.path.join(BUILD, name + ".o"), [src], [], hdrs))
     return out

@@ -393,8 +393,35 @@ PassArgs<T> step_args(gpe_engine_s* e, const StepPlan& sp) {
     return a;
 }
 
+// Launcher variant bit 5: the y-z plane plan (gpe_plane.hpp) where a plane
+// of ny = nz points fits in one workgroup's LDS: four passes per real-time
+// step (two plane passes, two x turn-arounds) instead of six.
+constexpr int kVariantPlane = 32;
+
+bool plane_eligible(const gpe_engine_s* e, int pin) {
+    return e->ny == e->nz && plane_length_ok(e->ny) && e->beta == 0.0 && e->qf == 0.0 && pin == 0 &&
+           e->interactions != INT_DIPOLAR;
+}
+
+bool plane_selected(const gpe_engine_s* e, int pin) { return (e->variant & kVariantPlane) && plane_eligible(e, pin); }
+
+template <typename T>
+void plane_run(gpe_engine_s* e, PassArgs<T> a, int mode, const void* src, void* dst, int flags) {
+    a.src = (const cplx<T>*)src;
+    a.dst = (cplx<T>*)dst;
+    a.tw = (const cplx<T>*)e->tw[2];  // ny = nz: the y and z tables are the same
+    a.twlo = (const cplx<T>*)e->twlo[2];
+    a.n1 = (unsigned)e->n;
+    a.flags = flags;
+    (void)launch_plane<T>(a, e->ny, mode, (unsigned)(e->batch * (size_t)e->nx), e->uses_hooks, e->stream);
+}
+
 template <typename T>
 void issue_open(gpe_engine_s* e, const StepPlan& sp) {
+    if (plane_selected(e, sp.pin)) {
+        plane_run<T>(e, step_args<T>(e, sp), PL_OPEN, nullptr, e->B, F_VSTEP1);
+        return;
+    }
     const bool qf = (e->qf != 0.0);
     const int imp = sp.pin ? F_IMPRINT : 0;
     PassArgs<T> a = step_args<T>(e, sp);
@@ -427,6 +454,18 @@ void kinetic_kz(gpe_engine_s* e, const PassArgs<T>& a, void* buf) {
 // one split step from time index e->it (the host counter is advanced by the caller)
 template <typename T>
 void issue_step(gpe_engine_s* e, const StepPlan& sp, bool last) {
+    if (plane_selected(e, sp.pin)) {
+        // x turn-around, plane pass, x turn-around, plane pass (gpe_plane.hpp)
+        const PassArgs<T> a = step_args<T>(e, sp);
+        void *A = e->A, *B = e->B;
+        run_pass<T>(e, a, {0, B, B, -1, +1, LOAD_PLAIN, MID_EXPT_SEP, STORE_PLAIN, 0, 0});
+        plane_run<T>(e, a, PL_VSTEP2, B, A, 0);
+        run_pass<T>(e, a, {0, A, A, -1, +1, LOAD_PLAIN, MID_EXPT_SEP, STORE_PLAIN, 0, 0});
+        if (!last) plane_run<T>(e, a, PL_VSTEP3_NEXT, A, B, F_NEXT_VSTEP1);
+        else plane_run<T>(e, a, PL_VSTEP3_LAST, A, nullptr, 0);
+        if (e->uses_hooks) launch_tick(e->itdev, e->stream);
+        return;
+    }
     const bool ite = (e->beta != 0.0);
     const bool qf = (e->qf != 0.0);
     const bool vortex = (sp.pin != 0);
@@ -1770,6 +1809,24 @@ int autotune_t(gpe_engine_s* e, int steps, int* chosen) {
         (void)hipEventElapsedTime(&ms, t0, t1);
         if (v == 0 || ms < best_ms) { best = v | plan; best_ms = ms; }
     }
+    // the plane plan where it applies (its x passes keep the best bits)
+    if (!err && plane_eligible(e, 0)) {
+        const int cand = best | kVariantPlane;
+        e->variant = cand;
+        StepPlan sp;
+        e->beta = 0.0;
+        issue_open<T>(e, sp);
+        issue_step<T>(e, sp, true);
+        (void)hipEventRecord(t0, e->stream);
+        issue_open<T>(e, sp);
+        for (int i = 0; i < steps; i++) issue_step<T>(e, sp, i == steps - 1);
+        (void)hipEventRecord(t1, e->stream);
+        e->beta = beta;
+        const hipError_t h = hipEventSynchronize(t1);
+        float ms = 0.f;
+        if (h != hipSuccess) err = (int)h;
+        else if (hipEventElapsedTime(&ms, t0, t1) == hipSuccess && ms < best_ms) { best = cand; best_ms = ms; }
+    }
     e->beta = beta;
     // the fused whole-step kernel where it applies, keeping the best passes
     // for the runs it does not cover
@@ -1819,7 +1876,7 @@ int gpe_engine_set_graph(gpe_engine_t e, int on) {
 int gpe_engine_set_variant(gpe_engine_t e, int variant) {
     if (!e) return GPE_ERR_NO_ENGINE;
     e->gen++;
-    if (variant < 0 || variant > 31) return GPE_ERR_ARG;
+    if (variant < 0 || variant > 63) return GPE_ERR_ARG;
     e->variant = variant;
     return 0;
 }

@@ -5,6 +5,7 @@
 #include "gpe_fft.hpp"
 #include "gpe_zwave.hpp"
 #include "gpe_fused.hpp"
+#include "gpe_plane.hpp"
 
 namespace gpe {
 
@@ -104,6 +105,13 @@ template <typename T>
 bool fused_length_ok(int L);
 template <typename T>
 int launch_fused_steps(const FusedArgs<T>& a, int L, unsigned batch, bool uh, hipStream_t s);
+
+// y-z plane passes (gpe_plane.hip, modes of PlaneMode in gpe_plane.hpp): the
+// plane lengths compiled (ny = nz = L), and one pass over `planes` planes
+// (lattices x nx) of L x L points; returns non-zero for a length without a kernel
+bool plane_length_ok(int L);
+template <typename T>
+int launch_plane(const PassArgs<T>& a, int L, int mode, unsigned planes, bool uh, hipStream_t s);
 
 // dipolar path (gpe_dipolar.hip)
 template <typename T>

@@ -424,7 +424,7 @@ class Ensemble:
         _check(lib().gpe_ensemble_autotune(self.h, int(steps)), "gpe_ensemble_autotune")
 
     def set_variant(self, group, variant):
-        """Launcher variant of one group (bits of Engine.set_variant; 16: fused whole steps)."""
+        """Launcher variant of one group (bits of Engine.set_variant; 16: fused whole steps, 32: y-z plane plan)."""
         _check(lib().gpe_ensemble_set_variant(self.h, int(group), int(variant)), "gpe_ensemble_set_variant")
 
     def groups(self):

@@ -102,8 +102,9 @@ def test_ensemble_mean_density_and_autotune():
     ens = make_ensemble(E.C64)
     ens.autotune(2)
     variants = [g[2] for g in ens.groups()]
-    # bits 0-2 of the passes, bit 3 as created, bit 4 the fused whole-step kernel
-    assert all(0 <= v <= 31 for v in variants)
+    # bits 0-2 of the passes, bit 3 as created, bit 4 the fused whole-step
+    # kernel, bit 5 the y-z plane plan
+    assert all(0 <= v <= 63 for v in variants)
     ens.evolve(3)
     for gi, (shape, count, _) in enumerate(ens.groups()):
         members = [i for i in range(len(SHAPES)) if ens.instance(i)[0] == gi]
