@@ -738,7 +738,12 @@ static int default_variant(const gpe_engine_s* e) {
         if (W <= 0) return 0;
         most = std::max(most, (e->n * e->batch / (size_t)N + (size_t)W - 1) / (size_t)W);
     }
-    return most < 4 * (size_t)cus ? 2 : 0;
+    int v = most < 4 * (size_t)cus ? 2 : 0;
+    // the y-z plane plan for 32 x 32 and 48 x 48 planes (measured in complex64:
+    // one 32^3 lattice 39k -> 51k steps/s, 48^3 28k -> 32k; 64 x 64 planes only
+    // win in batches, where the autotuner finds them)
+    if (e->ny == e->nz && (e->ny == 32 || e->ny == 48)) v |= kVariantPlane;
+    return v;
 }
 
 int create_impl(gpe_engine_s** out, const gpe_config_t* cfg, double alpha, double beta, double dt, double npart,

@@ -7,7 +7,7 @@ namespace {
 
 template <typename T, int L, bool UH>
 void launch_mode(const PassArgs<T>& a, int mode, unsigned planes, hipStream_t s) {
-    const dim3 g(planes), b(kPlaneThreads);
+    const dim3 g(planes), b(plane_threads<L>());
     switch (mode) {
     case PL_OPEN: hipLaunchKernelGGL((plane_kernel<T, L, PL_OPEN, UH>), g, b, 0, s, a); break;
     case PL_VSTEP2: hipLaunchKernelGGL((plane_kernel<T, L, PL_VSTEP2, UH>), g, b, 0, s, a); break;

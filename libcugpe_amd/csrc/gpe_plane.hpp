@@ -19,17 +19,24 @@
 // with rows padded to L + 1 elements; a Stockham stage takes pencil w and
 // butterfly j from b = j * L + w, so that the lanes of a wave walk pencils:
 // rows of stride L + 1 for the z transforms, consecutive elements for the y
-// transforms, both free of bank conflicts.  The physics is the per-element
-// code of the LDS-tile passes (load_op_m / mid_op_m).
+// transforms, both free of bank conflicts.  The physics runs in registers on
+// the outputs of the inverse y transform's last stage, in the chunked form of
+// the z-wave passes (per element with the user hooks), and the kinetic factor
+// K_y K_z / N and the store on those of the forward y transform's last stage.
 #pragma once
 
-#include "gpe_fft.hpp"
+#include "gpe_zwave.hpp"
 
 namespace gpe {
 
 enum PlaneMode { PL_OPEN = 0, PL_VSTEP2 = 1, PL_VSTEP3_NEXT = 2, PL_VSTEP3_LAST = 3 };
 
-constexpr int kPlaneThreads = 256;
+// threads of a plane workgroup: 512 for 64 x 64 planes (one butterfly of the
+// last stage per thread), 256 otherwise
+template <int L>
+constexpr int plane_threads() {
+    return L >= 64 ? 512 : 256;
+}
 
 template <typename T, int L>
 struct PlaneLds {
@@ -42,6 +49,7 @@ struct PlaneLds {
 // pencil w sits at w * WS + i * IS
 template <typename T, int L, int WS, int IS, int S, int R, int NS>
 __device__ __forceinline__ void plane_stage(cplx<T>* x, const TwE<T>* tw) {
+    constexpr int kPlaneThreads = plane_threads<L>();
     constexpr int M = L / R;
     constexpr int NB = L * M;
     constexpr int PER = (NB + kPlaneThreads - 1) / kPlaneThreads;
@@ -76,24 +84,54 @@ __device__ __forceinline__ void plane_stage(cplx<T>* x, const TwE<T>* tw) {
     __syncthreads();
 }
 
-template <typename T, int L, int WS, int IS, int S, int STAGE>
-__device__ __forceinline__ void plane_transform(cplx<T>* x, const TwE<T>* tw) {
+// stages [STAGE, LAST) of the plan of length L
+template <typename T, int L, int WS, int IS, int S, int STAGE, int LAST>
+__device__ __forceinline__ void plane_stages(cplx<T>* x, const TwE<T>* tw) {
     constexpr FftPlan P = plan_of(L, false, sizeof(T) == 4);
-    if constexpr (STAGE < P.ns) {
+    if constexpr (STAGE < LAST) {
         plane_stage<T, L, WS, IS, S, P.r[STAGE], plan_span(P, STAGE)>(x, tw);
-        plane_transform<T, L, WS, IS, S, STAGE + 1>(x, tw);
+        plane_stages<T, L, WS, IS, S, STAGE + 1, LAST>(x, tw);
     }
 }
 
-// 2-D transform of sign S over the plane: z (along rows), then y
+// radix of the last stage and the butterflies per thread there
+template <typename T, int L>
+struct PlaneLast {
+    static constexpr int kPlaneThreads = plane_threads<L>();
+    static constexpr FftPlan P = plan_of(L, false, sizeof(T) == 4);
+    static constexpr int R = P.r[P.ns - 1];
+    static constexpr int M = L / R;
+    static constexpr int NB = L * M;
+    static constexpr int PER = (NB + kPlaneThreads - 1) / kPlaneThreads;
+};
+
+// Last stage of a y transform (element iy of pencil iz at iz + iy (L + 1))
+// into registers: v[q][r] is element iy = j + r M of pencil iz = w, with
+// (w, j) of butterfly b = threadIdx.x + q * kPlaneThreads; ok[q] marks the
+// butterflies that exist.  Every thread's LDS reads are done on return.
 template <typename T, int L, int S>
-__device__ __forceinline__ void plane_fft2(cplx<T>* x, const TwE<T>* tw) {
-    constexpr int P = L + 1;
-    plane_transform<T, L, P, 1, S, 0>(x, tw);
-    plane_transform<T, L, 1, P, S, 0>(x, tw);
+__device__ __forceinline__ void plane_last_y(const cplx<T>* x, const TwE<T>* tw,
+                                             cplx<T> (&v)[PlaneLast<T, L>::PER][PlaneLast<T, L>::R]) {
+    using PL = PlaneLast<T, L>;
+    constexpr int kPlaneThreads = plane_threads<L>();
+    constexpr int R = PL::R, M = PL::M, P = L + 1;
+#pragma unroll
+    for (int q = 0; q < PL::PER; q++) {
+        const int b = threadIdx.x + q * kPlaneThreads;
+        if ((PL::NB % kPlaneThreads) == 0 || b < PL::NB) {
+            const int w = b % L, j = b / L;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[q][r] = x[w + (j + r * M) * P];
+#pragma unroll
+            for (int r = 1; r < R; r++) v[q][r] = tw_mul<S>(v[q][r], tw[j * r]);
+            dft<R, S>(v[q]);
+        }
+    }
+    __syncthreads();
 }
 
-// inputs of the physics at element (iy, iz) of plane (b, ix)
+// inputs of the per-element physics at element (iy, iz) of plane (b, ix)
+// (the user hook kernels)
 template <bool UH, typename T>
 __device__ __forceinline__ ElemIn<T> plane_elem(const PassArgs<T>& a, unsigned ix, unsigned iy, unsigned iz,
                                                 unsigned off, unsigned it, bool sep, T vscale, bool psi0, bool v1) {
@@ -115,13 +153,41 @@ __device__ __forceinline__ ElemIn<T> plane_elem(const PassArgs<T>& a, unsigned i
     return e;
 }
 
+// chunked V_ext over the R elements (iy = j + r M, iz = w) of a butterfly
+template <bool UH, typename T, int R, int M>
+__device__ __forceinline__ void plane_vext(const PassArgs<T>& a, unsigned ix, int w, int j, unsigned base, unsigned it,
+                                           bool sep, T vscale, T (&vx)[R]) {
+    const int nx = a.ph.nx, ny = a.ph.ny;
+    if (!UH && sep) {
+        const T vxz = a.ph.vtab[ix];
+        const T vz = a.ph.vtab[nx + ny + w];
+#pragma unroll
+        for (int r = 0; r < R; r++) vx[r] = vscale * (vxz + a.ph.vtab[nx + j + r * M] + vz);
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const unsigned iy = (unsigned)(j + r * M);
+            vx[r] = vext<UH>(a.ph, ix, iy, (unsigned)w, base + iy * (unsigned)a.ph.nz + (unsigned)w, it);
+        }
+    }
+}
+
 // One plane pass (MODE: PlaneMode) over every plane of the batch: workgroup
-// blockIdx.x takes plane (b, ix) = (blockIdx.x / nx, blockIdx.x % nx).
+// blockIdx.x takes plane (b, ix) = (blockIdx.x / nx, blockIdx.x % nx).  The
+// physics of the middle runs on the registers of the inverse y transform's
+// last stage, and the kinetic factor and the store on those of the forward y
+// transform's last stage.  UH: the user hook file (per-element ops).
 template <typename T, int L, int MODE, bool UH>
-__global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(plane_threads<L>()) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && L >= 64 ? 4 : 2)))
 void plane_kernel(const PassArgs<T> a) {
+    using PL = PlaneLast<T, L>;
+    constexpr int kPlaneThreads = plane_threads<L>();
     constexpr int P = L + 1;
     constexpr int NPT = L * L;
+    constexpr int PE = NPT / kPlaneThreads;  // elements per thread in the row-major loops
+    static_assert(NPT % kPlaneThreads == 0, "plane size");
+    constexpr int R = PL::R, M = PL::M, PER = PL::PER;
+    constexpr int NS = plan_of(L, false, sizeof(T) == 4).ns;
     __shared__ PlaneLds<T, L> S;
     for (int i = threadIdx.x; i < L; i += kPlaneThreads) {
         if constexpr (sizeof(T) == 4) {
@@ -139,43 +205,156 @@ void plane_kernel(const PassArgs<T> a) {
     const int vk = a.ph.vext;
     const bool sep = (vk == VEXT_HARMONIC || vk == VEXT_COSINE);
     const T vscale = (vk == VEXT_COSINE) ? a.ph.up[0] : (T)1;
+    const T cx = a.ph.c.x, cy = a.ph.c.y;
     double acc[3] = {0.0, 0.0, 0.0};
+    cplx<T> v[PER][R];
 
     if constexpr (MODE == PL_OPEN) {
-        // psi0 with the step's Vstep1 (the load op of the open z pass)
-        for (int e = threadIdx.x; e < NPT; e += kPlaneThreads) {
-            const unsigned iy = (unsigned)(e / L), iz = (unsigned)(e % L), off = base + (unsigned)e;
-            const ElemIn<T> in = plane_elem<UH>(a, ix, iy, iz, off, it, sep, vscale, false, false);
-            S.x[iy * P + iz] = load_op_m<LOAD_PSI0, UH>(a, in, a.psiA[off], acc);
+        // psi0 with the step's Vstep1 (orginal/gpe_engine.cu:723-772), row-major
+        cplx<T> u[PE];
+#pragma unroll
+        for (int k = 0; k < PE; k++) u[k] = a.psiA[base + (unsigned)(threadIdx.x + k * kPlaneThreads)];
+        if constexpr (!UH) {
+            T vx[PE], gv[PE];
+#pragma unroll
+            for (int k = 0; k < PE; k++) {
+                const int e = threadIdx.x + k * kPlaneThreads;
+                const unsigned iy = (unsigned)(e / L), iz = (unsigned)(e % L);
+                vx[k] = sep ? vscale * (a.ph.vtab[ix] + a.ph.vtab[nx + iy] + a.ph.vtab[nx + a.ph.ny + iz])
+                            : vext(a.ph, ix, iy, iz, base + (unsigned)e, it);
+            }
+            dedfdn_chunk(a.ph, u, gv);
+#pragma unroll
+            for (int k = 0; k < PE; k++) vx[k] = vx[k] + gv[k];
+            mul_exp_chunk(u, vx, cx, cy);
+        } else {
+#pragma unroll
+            for (int k = 0; k < PE; k++) {
+                const int e = threadIdx.x + k * kPlaneThreads;
+                const unsigned iy = (unsigned)(e / L), iz = (unsigned)(e % L), off = base + (unsigned)e;
+                const ElemIn<T> in = plane_elem<UH>(a, ix, iy, iz, off, it, sep, vscale, false, false);
+                u[k] = load_op_m<LOAD_PSI0, UH>(a, in, u[k], acc);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PE; k++) {
+            const int e = threadIdx.x + k * kPlaneThreads;
+            S.x[(e / L) * P + e % L] = u[k];
         }
         __syncthreads();
     } else {
-        for (int e = threadIdx.x; e < NPT; e += kPlaneThreads) S.x[(e / L) * P + e % L] = a.src[base + (unsigned)e];
+        // side inputs of the physics at this thread's last-stage elements, loaded early
+        cplx<T> side_c[MODE == PL_VSTEP2 ? PER : 1][MODE == PL_VSTEP2 ? R : 1];
+        T side_r[MODE == PL_VSTEP2 ? 1 : PER][MODE == PL_VSTEP2 ? 1 : R];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int b = threadIdx.x + q * kPlaneThreads;
+            if ((PL::NB % kPlaneThreads) == 0 || b < PL::NB) {
+                const int w = b % L, j = b / L;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const unsigned off = base + (unsigned)((j + r * M) * L + w);
+                    if constexpr (MODE == PL_VSTEP2) side_c[q][r] = ld_nt(a.psiA + off);
+                    else side_r[q][r] = ld_nt(a.vbuf + off);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PE; k++) {
+            const int e = threadIdx.x + k * kPlaneThreads;
+            S.x[(e / L) * P + e % L] = a.src[base + (unsigned)e];
+        }
         __syncthreads();
-        plane_fft2<T, L, +1>(S.x, S.tw);
-        for (int e = threadIdx.x; e < NPT; e += kPlaneThreads) {
-            const unsigned iy = (unsigned)(e / L), iz = (unsigned)(e % L), off = base + (unsigned)e;
-            cplx<T>& v = S.x[iy * P + iz];
+        plane_stages<T, L, P, 1, +1, 0, NS>(S.x, S.tw);      // inverse z
+        plane_stages<T, L, 1, P, +1, 0, NS - 1>(S.x, S.tw);  // inverse y but its last stage
+        plane_last_y<T, L, +1>(S.x, S.tw, v);
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int b = threadIdx.x + q * kPlaneThreads;
+            if (!((PL::NB % kPlaneThreads) == 0 || b < PL::NB)) continue;
+            const int w = b % L, j = b / L;
+            T vx[R], gv[R], lv[R];
+            if constexpr (!UH) plane_vext<false, T, R, M>(a, ix, w, j, base, it, sep, vscale, vx);
             if constexpr (MODE == PL_VSTEP2) {
-                v = mid_op_m<MID_VSTEP2, UH>(a, plane_elem<UH>(a, ix, iy, iz, off, it, sep, vscale, true, false), v,
-                                             acc);
+                // predictor end and corrector potential (orginal/gpe_engine.cu:774-801)
+                if constexpr (!UH) {
+                    cplx<T> p0[R];
+                    T V0[R];
+#pragma unroll
+                    for (int r = 0; r < R; r++) p0[r] = side_c[q][r];
+                    dedfdn_chunk(a.ph, p0, gv);
+#pragma unroll
+                    for (int r = 0; r < R; r++) V0[r] = vx[r] + gv[r];
+                    mul_exp_chunk(v[q], V0, cx, cy);  // phi3
+                    dedfdn_chunk(a.ph, v[q], gv);
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        lv[r] = (T)0.5 * (V0[r] + vx[r] + gv[r]);
+                        st_nt(a.vbuf + base + (unsigned)((j + r * M) * L + w), lv[r]);
+                    }
+                    mul_exp_chunk(p0, lv, cx, cy);
+#pragma unroll
+                    for (int r = 0; r < R; r++) v[q][r] = p0[r];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const unsigned iy = (unsigned)(j + r * M), off = base + iy * (unsigned)L + (unsigned)w;
+                        ElemIn<T> in = plane_elem<UH>(a, ix, iy, (unsigned)w, off, it, sep, vscale, false, false);
+                        in.aux = side_c[q][r];
+                        v[q][r] = mid_op_m<MID_VSTEP2, UH>(a, in, v[q][r], acc);
+                    }
+                }
             } else {
-                v = mid_op_m<MID_VSTEP3, UH>(a, plane_elem<UH>(a, ix, iy, iz, off, it, sep, vscale, false, true), v,
-                                             acc);
+                // corrector end (orginal/gpe_engine.cu:870-878), then the next step's Vstep1
+                if constexpr (!UH) {
+#pragma unroll
+                    for (int r = 0; r < R; r++) lv[r] = side_r[q][r];
+                    mul_exp_chunk(v[q], lv, cx, cy);  // psi1
+#pragma unroll
+                    for (int r = 0; r < R; r++) a.psiA[base + (unsigned)((j + r * M) * L + w)] = v[q][r];
+                    if constexpr (MODE == PL_VSTEP3_NEXT) {
+                        dedfdn_chunk(a.ph, v[q], gv);
+#pragma unroll
+                        for (int r = 0; r < R; r++) lv[r] = vx[r] + gv[r];
+                        mul_exp_chunk(v[q], lv, cx, cy);
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const unsigned iy = (unsigned)(j + r * M), off = base + iy * (unsigned)L + (unsigned)w;
+                        ElemIn<T> in = plane_elem<UH>(a, ix, iy, (unsigned)w, off, it, sep, vscale, false, false);
+                        in.raux = side_r[q][r];
+                        v[q][r] = mid_op_m<MID_VSTEP3, UH>(a, in, v[q][r], acc);
+                    }
+                }
+            }
+            if constexpr (MODE != PL_VSTEP3_LAST) {
+#pragma unroll
+                for (int r = 0; r < R; r++) S.x[w + (j + r * M) * P] = v[q][r];
             }
         }
         if constexpr (MODE == PL_VSTEP3_LAST) return;  // psi1 is in psi (A)
         __syncthreads();
     }
-    plane_fft2<T, L, -1>(S.x, S.tw);
-    // K_y K_z / N of the split kinetic factor, as the y turn-around pass applies it
-    for (int e = threadIdx.x; e < NPT; e += kPlaneThreads) {
-        Coord c;
-        c.iy = (unsigned)(e / L);
-        c.iz = (unsigned)(e % L);
-        c.ix = ix;
-        c.off = base + (unsigned)e;
-        a.dst[c.off] = mid_op_lean_k<MK_SEP, false, T, 1>(a, c, S.x[c.iy * P + c.iz]);
+    plane_stages<T, L, P, 1, -1, 0, NS>(S.x, S.tw);      // forward z
+    plane_stages<T, L, 1, P, -1, 0, NS - 1>(S.x, S.tw);  // forward y but its last stage
+    plane_last_y<T, L, -1>(S.x, S.tw, v);
+    // K_y K_z / N of the split kinetic factor, as the y turn-around pass applies it, and the store
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int b = threadIdx.x + q * kPlaneThreads;
+        if ((PL::NB % kPlaneThreads) == 0 || b < PL::NB) {
+            const int w = b % L, j = b / L;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                Coord c;
+                c.ix = ix;
+                c.iy = (unsigned)(j + r * M);
+                c.iz = (unsigned)w;
+                c.off = base + c.iy * (unsigned)L + c.iz;
+                a.dst[c.off] = mid_op_lean_k<MK_SEP, false, T, 1>(a, c, v[q][r]);
+            }
+        }
     }
 }
 
