@@ -13,6 +13,15 @@ A ``.dftc`` file is raw complex128 frames in C order ``[nom][nx][ny][nz]``
 The reference's Python readers (python/read_dftc_info.py:17-38,
 python/gpe_dftcutils.py:43-56) take the header form and return the density
 as gamma |psi|^2 a_ho^3 with a_ho = 1/sqrt(omega_x) and gamma = 2 for fermions.
+
+Side files of a run (python/gpe_dftcutils.py:79-145, 194-218), written here
+by the ``write_*`` / ``*Writer`` helpers from engine outputs:
+
+- ``<file>.energy``: a header line, then one text row per measurement:
+  time, E_tot, E_kin, E_int, E_ext and, in imaginary time, mu and the energy
+  change, in real time one more column;
+- ``<file>.trajectory``: a header line, then rows t x y z (vortex positions);
+- ``<file>.currents``: raw float64 frames ``[nom][3][nx][ny][nz]`` (j_x, j_y, j_z).
 """
 import math
 import os
@@ -135,3 +144,131 @@ def density_phase(dftc, frame=0):
     gamma = 2.0 if "fermions" in info.get("system", "") else 1.0
     psi = np.asarray(read_frames(dftc)[frame])
     return gamma * np.abs(psi) ** 2 * a_ho ** 3, np.angle(psi)
+
+
+# ------------------------------------------------------------ side files
+
+
+def write_energy(dftc, rows, mode="ite"):
+    """Write ``<dftc>.energy``: rows of (time, etot, ekin, eint, eext, chem,
+    diff) in imaginary time or (time, etot, ekin, eint, eext, comp) in real time."""
+    cols = ENERGY_COLUMNS[mode]
+    with open(dftc + ".energy", "w") as f:
+        f.write("# " + " ".join(cols) + "\n")
+        for r in rows:
+            if len(r) != len(cols):
+                raise ValueError("energy row of %d values, %s mode has %d" % (len(r), mode, len(cols)))
+            f.write(" ".join("%.15e" % float(x) for x in r) + "\n")
+
+
+ENERGY_COLUMNS = {"ite": ("time", "etot", "ekin", "eint", "eext", "chem", "diff"),
+                  "rte": ("time", "etot", "ekin", "eint", "eext", "comp")}
+
+
+def read_energy(dftc, mode=None):
+    """``<dftc>.energy`` as a dict of columns, as gpe_dftcutils.get_energy: the
+    mode is 'rte' when the file name holds '_rte', 'ite' otherwise."""
+    if mode is None:
+        mode = "rte" if "_rte" in dftc else "ite"
+    data = np.loadtxt(dftc + ".energy", skiprows=1, ndmin=2)
+    return {name: data[:, k] for k, name in enumerate(ENERGY_COLUMNS[mode])}
+
+
+def write_trajectory(dftc, rows):
+    """Write ``<dftc>.trajectory``: rows of (t, x, y, z)."""
+    with open(dftc + ".trajectory", "w") as f:
+        f.write("# t x y z\n")
+        for t, x, y, z in rows:
+            f.write("%.15e %.15e %.15e %.15e\n" % (t, x, y, z))
+
+
+def read_trajectory(dftc):
+    """(t, x, y, z) columns of ``<dftc>.trajectory``, as gpe_dftcutils.get_trajectory."""
+    data = np.loadtxt(dftc + ".trajectory", skiprows=1, ndmin=2)
+    return data[:, 0], data[:, 1], data[:, 2], data[:, 3]
+
+
+def trajectory_in_xy(T, X, Y, Z, slice_z):
+    """Points of a trajectory in the plane z = slice_z: (t, x, y)."""
+    m = Z == slice_z
+    return T[m], X[m], Y[m]
+
+
+def trajectory_at_t(T, X, Y, Z, slice_t):
+    """Points of a trajectory at time slice_t: (x, y, z)."""
+    m = T == slice_t
+    return X[m], Y[m], Z[m]
+
+
+class CurrentsWriter:
+    """Appends (j_x, j_y, j_z) frames, float64 [3][nx][ny][nz] each, to ``<dftc>.currents``."""
+
+    def __init__(self, dftc, shape, append=False):
+        self.shape = tuple(int(s) for s in shape)
+        self.f = open(dftc + ".currents", "ab" if append else "wb")
+
+    def append(self, jx, jy, jz):
+        for j in (jx, jy, jz):
+            a = np.ascontiguousarray(np.asarray(j, dtype=np.float64))
+            if a.shape != self.shape:
+                raise ValueError("current shape %s, file %s" % (a.shape, self.shape))
+            a.tofile(self.f)
+        self.f.flush()
+
+    def close(self):
+        if self.f:
+            self.f.close()
+            self.f = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def read_currents(dftc, frame=0):
+    """(j_x, j_y, j_z) of one frame of ``<dftc>.currents``, as gpe_dftcutils.get_currents."""
+    info = read_info(dftc)
+    n = info["nx"] * info["ny"] * info["nz"]
+    data = np.memmap(dftc + ".currents", dtype=np.float64, mode="r")[frame * 3 * n:(frame + 1) * 3 * n]
+    v = np.asarray(data).reshape(3, info["nx"], info["ny"], info["nz"])
+    return v[0], v[1], v[2]
+
+
+# ------------------------------------------------------------ slices and grids
+# (python/gpe_dftcutils.py:149-191)
+
+def get_slice_x(data, y_slice=32, z_slice=32):
+    return data[:, y_slice, z_slice]
+
+
+def get_slice_y(data, x_slice=32, z_slice=32):
+    return data[x_slice, :, z_slice]
+
+
+def get_slice_z(data, x_slice=32, y_slice=32):
+    return data[x_slice, y_slice, :]
+
+
+def get_slice_xy(data, z_slice=32):
+    return data[:, :, z_slice]
+
+
+def get_slice_xz(data, y_slice=32):
+    return data[:, y_slice, :]
+
+
+def get_slice_yz(data, x_slice=32):
+    return data[x_slice, :, :]
+
+
+def create_grid(Nx=8, Ny=8, Nz=8, scale=1.0):
+    """Lattice coordinates i - n/2 per axis, times scale."""
+    return tuple((np.arange(n, dtype=np.float64) - n / 2.0) * scale for n in (Nx, Ny, Nz))
+
+
+def create_meshgrid(Nx=8, Ny=8, Nz=0, scale=1.0):
+    """Meshgrid (indexing 'ij') of the axes whose size is not 0."""
+    axes = [a for a, n in zip(create_grid(Nx, Ny, Nz, scale), (Nx, Ny, Nz)) if n != 0]
+    return tuple(np.meshgrid(*axes, indexing="ij"))

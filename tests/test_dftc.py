@@ -71,3 +71,43 @@ def test_python_writer_c_reader(tmp_path, info_tool):
     assert [int(v) for v in out[:3]] == [8, 12, 20]
     assert float(out[3]) == 0.5 and int(out[4]) == 3
     assert [float(v) for v in out[5:]] == [0.75, 1.25, 0.5, 2.0, 321.0]
+
+
+def test_side_files_and_slices(tmp_path):
+    """.energy / .trajectory / .currents side files (python/gpe_dftcutils.py:79-145,
+    194-218) written and read back, slices and grids."""
+    path = str(tmp_path / "run_rte.dftc")
+    shape = (4, 3, 5)
+    dftc.write_info(path, *shape, 0.01, 2, 0.0, 1.0, 1.0, 0.0, 10.0, total_time=0.02, system="bosons", mode="rte")
+    rows = [(0.01 * k, -1.0 - k, 0.5, -1.7, 0.2 + k, 1e-9 * k) for k in range(3)]
+    dftc.write_energy(path, rows, mode="rte")
+    e = dftc.read_energy(path)  # mode from the file name
+    assert set(e) == {"time", "etot", "ekin", "eint", "eext", "comp"}
+    assert np.array_equal(e["etot"], np.array([r[1] for r in rows]))
+    with pytest.raises(ValueError):
+        dftc.write_energy(path, [(0.0, 1.0)], mode="ite")
+    traj = [(0.0, 1.0, 2.0, 0.0), (0.1, 1.5, 2.5, 0.0), (0.1, 1.0, 2.0, 3.0)]
+    dftc.write_trajectory(path, traj)
+    T, X, Y, Z = dftc.read_trajectory(path)
+    assert np.array_equal(X, [1.0, 1.5, 1.0])
+    t, x, y = dftc.trajectory_in_xy(T, X, Y, Z, 0.0)
+    assert np.array_equal(t, [0.0, 0.1]) and np.array_equal(y, [2.0, 2.5])
+    x, y, z = dftc.trajectory_at_t(T, X, Y, Z, 0.1)
+    assert np.array_equal(z, [0.0, 3.0])
+    rng = np.random.default_rng(2)
+    js = [[rng.standard_normal(shape) for _ in range(3)] for _ in range(2)]
+    with dftc.CurrentsWriter(path, shape) as w:
+        for j in js:
+            w.append(*j)
+    assert os.path.getsize(path + ".currents") == 2 * 3 * 8 * int(np.prod(shape))
+    for k in range(2):
+        got = dftc.read_currents(path, frame=k)
+        for a, b in zip(got, js[k]):
+            assert np.array_equal(a, b)
+    d = np.arange(np.prod(shape)).reshape(shape)
+    assert np.array_equal(dftc.get_slice_x(d, 1, 2), d[:, 1, 2])
+    assert np.array_equal(dftc.get_slice_yz(d, 3), d[3])
+    x, y, z = dftc.create_grid(4, 3, 5, scale=0.5)
+    assert np.array_equal(x, [-1.0, -0.5, 0.0, 0.5]) and np.array_equal(y, [-0.75, -0.25, 0.25])
+    X, Y = dftc.create_meshgrid(4, 3, 0)
+    assert X.shape == (4, 3) and np.array_equal(X[:, 0], 2 * x) and np.array_equal(Y[0], 2 * y)
