@@ -235,7 +235,10 @@ const char *gpe_build_info(void);
  * lattice, the lower bits serving the runs it does not cover, and bit 5
  * runs y-z planes of ny = nz = 32, 48 or 64 points in one pass (z and y
  * transforms with the physics between them: four passes per real-time step
- * instead of six; no quantum friction, no pinning).  autotune times
+ * instead of six; no quantum friction, no pinning), and bit 6 runs the
+ * strided turn-around passes with the alternative radix schedule of their
+ * length (the radices in the other order, or the first one split).  autotune
+ * times
  * `steps` real-time steps per choice on the current state (restored
  * afterwards) and keeps the fastest. */
 int gpe_engine_autotune(gpe_engine_t e, int steps, int *variant);

@@ -397,6 +397,9 @@ PassArgs<T> step_args(gpe_engine_s* e, const StepPlan& sp) {
 // of ny = nz points fits in one workgroup's LDS: four passes per real-time
 // step (two plane passes, two x turn-arounds) instead of six.
 constexpr int kVariantPlane = 32;
+// Launcher variant bit 6: the alternative radix schedule (alt_plan) in the
+// strided turn-arounds of the split kinetic factor.
+constexpr int kVariantAltPlan = 64;
 
 bool plane_eligible(const gpe_engine_s* e, int pin) {
     return e->ny == e->nz && plane_length_ok(e->ny) && e->beta == 0.0 && e->qf == 0.0 && pin == 0 &&
@@ -1776,9 +1779,11 @@ int ens_evolve_t(gpe_ensemble_s* s, int nt) {
 // Times `steps` real-time steps of one engine for each launcher variant
 // (bit 0: z passes in LDS tiles instead of wave registers, bit 1: half-width
 // strided tiles, bit 2: pipelined strided passes) and keeps the fastest; bit 3
-// (the unsplit kinetic plan) is kept as set.  Where the fused whole-step kernel
-// applies, bit 4 is added when it beats the fastest passes.  The state is saved
-// and restored.
+// (the unsplit kinetic plan) is kept as set.  Then the alternative radix
+// schedule of the strided turn-arounds (bit 6) and the y-z plane plan (bit 5)
+// are tried on top of the fastest, and where the fused whole-step kernel
+// applies, bit 4 is added when it beats the rest.  The state is saved and
+// restored.
 template <typename T>
 int autotune_t(gpe_engine_s* e, int steps, int* chosen) {
     if (steps < 1) steps = 1;
@@ -1813,6 +1818,25 @@ int autotune_t(gpe_engine_s* e, int steps, int* chosen) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, t0, t1);
         if (v == 0 || ms < best_ms) { best = v | plan; best_ms = ms; }
+    }
+    // the alternative radix schedule of the strided turn-arounds (variant bit 6)
+    const bool c64 = e->precision == GPE_PRECISION_C64;
+    if (!err && !(best & 8) && (has_alt_plan(e->nx, c64) || has_alt_plan(e->ny, c64))) {
+        const int cand = best | kVariantAltPlan;
+        e->variant = cand;
+        StepPlan sp;
+        e->beta = 0.0;
+        issue_open<T>(e, sp);
+        issue_step<T>(e, sp, true);
+        (void)hipEventRecord(t0, e->stream);
+        issue_open<T>(e, sp);
+        for (int i = 0; i < steps; i++) issue_step<T>(e, sp, i == steps - 1);
+        (void)hipEventRecord(t1, e->stream);
+        e->beta = beta;
+        const hipError_t h = hipEventSynchronize(t1);
+        float ms = 0.f;
+        if (h != hipSuccess) err = (int)h;
+        else if (hipEventElapsedTime(&ms, t0, t1) == hipSuccess && ms < best_ms) { best = cand; best_ms = ms; }
     }
     // the plane plan where it applies (its x passes keep the best bits)
     if (!err && plane_eligible(e, 0)) {
@@ -1881,7 +1905,7 @@ int gpe_engine_set_graph(gpe_engine_t e, int on) {
 int gpe_engine_set_variant(gpe_engine_t e, int variant) {
     if (!e) return GPE_ERR_NO_ENGINE;
     e->gen++;
-    if (variant < 0 || variant > 63) return GPE_ERR_ARG;
+    if (variant < 0 || variant > 127) return GPE_ERR_ARG;
     e->variant = variant;
     return 0;
 }

@@ -881,8 +881,45 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // transform, so the mid op and the hand-over stay in registers.  The final
 // registers go straight to global memory.
 
-constexpr FftPlan plan_of(int N, bool rev, bool c64) {
+// The alternative radix schedule of a length (launcher variant bit 6, chosen
+// by the autotuner for the strided turn-around passes): the radices in the
+// other order, or, where that is the same plan, the first radix split
+// (64 = 4.16, 256 = 4.4.16, 512 = 4.8.16 for complex128); lengths of one
+// stage have none.
+constexpr FftPlan alt_plan(int N, bool c64) {
     FftPlan p = make_plan(N, c64);
+    bool palindrome = true;
+    for (int i = 0; i < p.ns / 2; i++) palindrome = palindrome && (p.r[i] == p.r[p.ns - 1 - i]);
+    if (p.ns < 2) return p;
+    if (!palindrome) {
+        for (int i = 0; i < p.ns / 2; i++) {
+            const int t = p.r[i];
+            p.r[i] = p.r[p.ns - 1 - i];
+            p.r[p.ns - 1 - i] = t;
+        }
+        return p;
+    }
+    FftPlan q{0, {0, 0, 0, 0, 0, 0, 0, 0}};
+    if (N == 64) { q.ns = 2; q.r[0] = 4; q.r[1] = 16; return q; }
+    if (N == 256) { q.ns = 3; q.r[0] = 4; q.r[1] = 4; q.r[2] = 16; return q; }
+    if (N == 512) { q.ns = 3; q.r[0] = 4; q.r[1] = 8; q.r[2] = 16; return q; }
+    return p;
+}
+
+constexpr bool same_plan(const FftPlan& a, const FftPlan& b) {
+    if (a.ns != b.ns) return false;
+    for (int i = 0; i < a.ns; i++)
+        if (a.r[i] != b.r[i]) return false;
+    return true;
+}
+
+// whether a length has an alternative schedule
+constexpr bool has_alt_plan(int N, bool c64) { return !same_plan(alt_plan(N, c64), make_plan(N, c64)); }
+
+// plan of a transform: PV 0 the default schedule, 1 the alternative; rev
+// reverses the radix order (the second transform of a turn-around pass)
+constexpr FftPlan plan_of(int N, bool rev, bool c64, int pv = 0) {
+    FftPlan p = pv ? alt_plan(N, c64) : make_plan(N, c64);
     if (rev)
         for (int i = 0; i < p.ns / 2; i++) {
             int t = p.r[i];
@@ -1005,13 +1042,13 @@ __device__ __forceinline__ void first_stage_regs(cplx<T> (&v)[PER][R], cplx<T>* 
     }
 }
 
-// stages [STAGE, LAST) of plan (N, REV) in LDS
-template <typename T, int N, int W, bool CONTIG, int S, bool REV, int STAGE, int LAST>
+// stages [STAGE, LAST) of plan (N, REV, PV) in LDS
+template <typename T, int N, int W, bool CONTIG, int S, bool REV, int STAGE, int LAST, int PV = 0>
 __device__ __forceinline__ void lds_stages(cplx<T>* lds, const TwE<T>* tw) {
     if constexpr (STAGE < LAST) {
-        constexpr FftPlan P = plan_of(N, REV, sizeof(T) == 4);
+        constexpr FftPlan P = plan_of(N, REV, sizeof(T) == 4, PV);
         fft_stage<T, N, W, CONTIG, S, P.r[STAGE], plan_span(P, STAGE)>(lds, tw);
-        lds_stages<T, N, W, CONTIG, S, REV, STAGE + 1, LAST>(lds, tw);
+        lds_stages<T, N, W, CONTIG, S, REV, STAGE + 1, LAST, PV>(lds, tw);
     }
 }
 
@@ -1117,25 +1154,25 @@ constexpr int lean_w() {
 // What follows the first stage of a lean pass of two or more stages (the first
 // stage's output is in LDS): the remaining stages, the mid op, the optional
 // second transform and the store of tile p0.
-template <typename T, int N, int W, bool CONTIG, int S1, int S2, int MK, bool CJ, int AX>
+template <typename T, int N, int W, bool CONTIG, int S1, int S2, int MK, bool CJ, int AX, int PV = 0>
 __device__ __forceinline__ void lean_tail(const PassArgs<T>& a, unsigned p0, cplx<T>* lds, const TwE<T>* tw) {
-    constexpr FftPlan P = plan_of(N, false, sizeof(T) == 4);
+    constexpr FftPlan P = plan_of(N, false, sizeof(T) == 4, PV);
     constexpr int NSG = P.ns;
     constexpr int RL = P.r[NSG - 1];
     constexpr int PERL = (W * (N / RL) + kThreads - 1) / kThreads;
     cplx<T> v[PERL][RL];
-    lds_stages<T, N, W, CONTIG, S1, false, 1, NSG - 1>(lds, tw);
+    lds_stages<T, N, W, CONTIG, S1, false, 1, NSG - 1, PV>(lds, tw);
     last_stage_regs<T, N, W, CONTIG, S1, RL, PERL>(lds, tw, v);
     if constexpr (MK >= 0) mid_regs_k<MK, CJ, T, N, W, CONTIG, RL, PERL, AX>(a, p0, v);
     if constexpr (S2 != 0) {
-        constexpr FftPlan Q = plan_of(N, true, sizeof(T) == 4);
+        constexpr FftPlan Q = plan_of(N, true, sizeof(T) == 4, PV);
         static_assert(Q.r[0] == RL, "reversed plan must start with the last radix");
         constexpr int RF = Q.r[NSG - 1];
         constexpr int PERF = (W * (N / RF) + kThreads - 1) / kThreads;
         __syncthreads();  // the first transform's last-stage reads are done
         first_stage_regs<T, N, W, CONTIG, S2, RL, PERL>(v, lds);
         __syncthreads();
-        lds_stages<T, N, W, CONTIG, S2, true, 1, NSG - 1>(lds, tw);
+        lds_stages<T, N, W, CONTIG, S2, true, 1, NSG - 1, PV>(lds, tw);
         cplx<T> u[PERF][RF];
         last_stage_regs<T, N, W, CONTIG, S2, RF, PERF>(lds, tw, u);
         store_regs<T, N, W, CONTIG, RF, PERF, AX>(a, p0, u);
@@ -1145,10 +1182,10 @@ __device__ __forceinline__ void lean_tail(const PassArgs<T>& a, unsigned p0, cpl
 }
 
 // MK: compile-time kind of the mid op (MidKind), -1 for none; CJ conjugates it
-template <typename T, int N, bool CONTIG, int S1, int S2, int WD, int MK, bool CJ, int AX>
+template <typename T, int N, bool CONTIG, int S1, int S2, int WD, int MK, bool CJ, int AX, int PV = 0>
 __device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, const TwE<T>* tw) {
     constexpr int W = lean_w<T, N, WD>();
-    constexpr FftPlan P = plan_of(N, false, sizeof(T) == 4);
+    constexpr FftPlan P = plan_of(N, false, sizeof(T) == 4, PV);
     constexpr int NSG = P.ns;
     constexpr int RL = P.r[NSG - 1];
     constexpr int PERL = (W * (N / RL) + kThreads - 1) / kThreads;
@@ -1167,7 +1204,7 @@ __device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, co
     } else {
         first_stage_global<T, N, W, CONTIG, false, S1, P.r[0], AX>(a, p0, lds, nullptr);
         __syncthreads();
-        lean_tail<T, N, W, CONTIG, S1, S2, MK, CJ, AX>(a, p0, lds, tw);
+        lean_tail<T, N, W, CONTIG, S1, S2, MK, CJ, AX, PV>(a, p0, lds, tw);
     }
 }
 
@@ -1458,7 +1495,7 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
 }
 
 template <typename T, int N, bool CONTIG, bool FULL, int SHAPE, int WD = 1, int MK = -1, bool CJ = false, int AX = -1,
-          bool UH = false>
+          bool UH = false, int PV = 0>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
 void fft_pass_kernel(const PassArgs<T> a) {
     __shared__ TwE<T> tw[N];
@@ -1477,7 +1514,7 @@ void fft_pass_kernel(const PassArgs<T> a) {
     if constexpr (!FULL) {
         constexpr int W = lean_w<T, N, WD>();
         __shared__ cplx<T> lds[lds_elems<N, W, CONTIG>()];
-        lean_body<T, N, CONTIG, S1, S2, WD, MK, CJ, (CONTIG ? 2 : AX)>(a, lds, tw);
+        lean_body<T, N, CONTIG, S1, S2, WD, MK, CJ, (CONTIG ? 2 : AX), PV>(a, lds, tw);
     } else {
         __shared__ FullLds<T, N, CONTIG> L;
         __shared__ double red[kThreads / 64];

@@ -39,14 +39,24 @@ unsigned launch_fft_pass_n_impl(const PassArgs<T>& a, hipStream_t s) {
 #define GPE_LAUNCH_MID(WD, MK, CJ)                                                                            \
     hipLaunchKernelGGL((fft_pass_kernel<T, NN, false, false, SHAPE_FI, WD, MK, CJ, 0>), dim3(blocks),          \
                        dim3(kThreads), 0, s, a)
-#define GPE_LAUNCH_SEP(WD)                                                                                     \
+// (PV: the alternative radix schedule of variant bit 6, for lengths that have one)
+#define GPE_LAUNCH_SEP_PV(WD, PV)                                                                              \
     do {                                                                                                       \
         if (a.axis == 0)                                                                                       \
-            hipLaunchKernelGGL((fft_pass_kernel<T, NN, false, false, SHAPE_FI, WD, MK_SEP, false, 0>),         \
+            hipLaunchKernelGGL((fft_pass_kernel<T, NN, false, false, SHAPE_FI, WD, MK_SEP, false, 0, false, PV>), \
                                dim3(blocks), dim3(kThreads), 0, s, a);                                         \
         else                                                                                                   \
-            hipLaunchKernelGGL((fft_pass_kernel<T, NN, false, false, SHAPE_FI, WD, MK_SEP, false, 1>),         \
+            hipLaunchKernelGGL((fft_pass_kernel<T, NN, false, false, SHAPE_FI, WD, MK_SEP, false, 1, false, PV>), \
                                dim3(blocks), dim3(kThreads), 0, s, a);                                         \
+    } while (0)
+#define GPE_LAUNCH_SEP(WD)                                                                                     \
+    do {                                                                                                       \
+        if constexpr (has_alt_plan(NN, sizeof(T) == 4)) {                                                      \
+            if (a.variant & 64) GPE_LAUNCH_SEP_PV(WD, 1);                                                      \
+            else GPE_LAUNCH_SEP_PV(WD, 0);                                                                     \
+        } else {                                                                                               \
+            GPE_LAUNCH_SEP_PV(WD, 0);                                                                          \
+        }                                                                                                      \
     } while (0)
     int mk = -1;
     bool cj = false;
@@ -147,6 +157,7 @@ unsigned launch_fft_pass_n_impl(const PassArgs<T>& a, hipStream_t s) {
     }
 #undef GPE_LAUNCH_S
 #undef GPE_LAUNCH_SEP
+#undef GPE_LAUNCH_SEP_PV
 #undef GPE_LAUNCH_MID
 #undef GPE_LAUNCH
     return blocks;

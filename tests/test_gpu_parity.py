@@ -299,12 +299,15 @@ def test_pinned_variants_match_oracle(oracle, variant, mode):
     g.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 8, 11])
-@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 8, 64), (16, 96, 8), (8, 12, 48), (16, 8, 24)])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 8, 11, 64, 66])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 8, 64), (16, 96, 8), (8, 12, 48), (16, 8, 24), (64, 128, 12),
+                                   (256, 8, 8), (8, 512, 8)])
 def test_launcher_variants_match_oracle(oracle, variant, shape):
     """Every launcher choice of the autotuner (z passes in wave registers or LDS
-    tiles, full or half-width strided tiles) follows the oracle's step, with
-    the kinetic factor split over the y and x turn-arounds (the default) and
+    tiles, full or half-width strided tiles, the alternative radix schedules
+    of bit 6: 32 = 4.8, 48 = 3.16, 64 = 4.16, 96 = 3.2.16, 128 = 8.16,
+    256 = 4.4.16, 512 = 16.32 / 4.8.16) follows the oracle's step, with the
+    kinetic factor split over the y and x turn-arounds (the default) and
     unsplit in the x turn-around (bit 3)."""
     say("variant", variant, shape)
     for mode in ("rte", "ite"):
