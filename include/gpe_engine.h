@@ -48,6 +48,11 @@ typedef struct __attribute__((aligned(16))) {
     double x, y;
 } cuCplx;
 typedef cuCplx Complex; /* orginal/gpe_engine.h:171 */
+/* the name the orginal drivers use (test_compare/gpe_imag.c:33-35 typedefs
+ * cuCplx from it when built without -DDEV) */
+#ifndef GPE_NO_CUDOUBLECOMPLEX
+typedef cuCplx cuDoubleComplex;
+#endif
 #ifndef GPE_NO_CPLX
 typedef double _Complex cplx; /* C99 complex128, dev/gpe_complex.cuh:36 */
 #endif

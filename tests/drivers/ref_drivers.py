@@ -31,8 +31,13 @@ RUN = {"gpe_imag_dev": ("test_compare/gpe_imag.c", LATTICE), "gpe_real_dev": ("t
 
 
 def port_driver(src):
-    """The integration notes' edits, applied to a driver's source text."""
-    s = re.sub(r"^[ \t]*#include[ \t]*<(?:cuda|cuda_runtime|cufft)\.h>[ \t]*\n", "", src, flags=re.M)
+    """The integration notes' edits, applied to a driver's source text: the
+    first CUDA header include becomes the engine header (which defines the
+    complex types the drivers name before their own include of it), the other
+    CUDA header includes go."""
+    cuda_inc = r"^[ \t]*#include[ \t]*<(?:cuda|cuda_runtime|cufft)\.h>[ \t]*\n"
+    s = re.sub(cuda_inc, '#include "gpe_engine.h"\n', src, count=1, flags=re.M)
+    s = re.sub(cuda_inc, "", s, flags=re.M)
     s = re.sub(r"\bcudaError(?:_t)?(\s+err\s*;)", r"int\1", s)
     s = s.replace("cudaSetDevice(", "gpe_set_device(")
     s = re.sub(r"cudaHostAlloc\(\s*(?:\(void\s*\*\*\)\s*)?(&\s*\w+)\s*,\s*([^,]+?)\s*,\s*cudaHostAllocDefault\s*\)",
@@ -42,8 +47,9 @@ def port_driver(src):
     return s
 
 
-def compile_driver(rel_src, exe, lattice=LATTICE, workdir=None):
-    """Port and compile one driver; returns (returncode, compiler output)."""
+def compile_driver(rel_src, exe, lattice=LATTICE, workdir=None, dev=True):
+    """Port and compile one driver (dev: with the reference's -DDEV, else the
+    orginal spelling); returns (returncode, compiler output)."""
     cxx = shutil.which("g++")
     if not cxx:
         return 1, "no g++"
@@ -54,7 +60,7 @@ def compile_driver(rel_src, exe, lattice=LATTICE, workdir=None):
         f.write(src)
     nx, ny, nz = lattice
     cmd = [cxx, "-std=gnu++14", "-O2", "-fopenmp", "-w", "-DNX=%d" % nx, "-DNY=%d" % ny, "-DNZ=%d" % nz,
-           "-DINTERACTIONS=0", "-DVEXT=0", "-DDEV", "-I" + os.path.join(ROOT, "include"), c, "-o", exe,
+           "-DINTERACTIONS=0", "-DVEXT=0"] + (["-DDEV"] if dev else []) + ["-I" + os.path.join(ROOT, "include"), c, "-o", exe,
            "-L" + os.path.dirname(LIB), "-lgpe_hip", "-Wl,-rpath,$ORIGIN/../../../libcugpe_amd",
            "-Wl,-rpath-link,/opt/rocm/lib"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)

@@ -32,14 +32,19 @@ def test_port_edits_touch_only_the_cuda_runtime_calls():
            "err=cudaHostAlloc( &psi_init , sizeof(cplx)*nxyz, cudaHostAllocDefault );\n"
            "gpe_exec( gpe_evolve_qf(100), ierr );\ncudaFreeHost(psi);\n")
     out = port_driver(src)
-    assert out == ("#include <math.h>\nint err;\nerr=gpe_set_device( device );\nif(err != 0) return 1;\n"
+    assert out == ("#include \"gpe_engine.h\"\n#include <math.h>\nint err;\nerr=gpe_set_device( device );\n"
+                   "if(err != 0) return 1;\n"
                    "err=gpe_host_alloc((void**)&psi, sizeof(cuCplx)*nxyz);\n"
                    "err=gpe_host_alloc((void**)&psi_init, sizeof(cplx)*nxyz);\n"
                    "gpe_exec( gpe_evolve_qf(100), ierr );\ngpe_host_free(psi);\n")
 
 
+@pytest.mark.parametrize("dev", [True, False], ids=["dev", "orginal"])
 @pytest.mark.parametrize("driver", DRIVERS)
-def test_reference_driver_builds_against_the_engine(driver, tmp_path):
+def test_reference_driver_builds_against_the_engine(driver, dev, tmp_path):
+    """Both spellings of the drivers: -DDEV (the dev/ library) and without it,
+    where gpe_imag.c / gpe_real.c typedef cuCplx from cuDoubleComplex
+    (test_compare/gpe_imag.c:33-35)."""
     path = os.path.join(REF, driver)
     if not os.path.exists(path):
         pytest.skip("reference tree not present")
@@ -53,7 +58,7 @@ def test_reference_driver_builds_against_the_engine(driver, tmp_path):
     c.write_text(src)
     exe = str(tmp_path / "driver")
     cmd = [cxx, "-std=gnu++14", "-O1", "-fopenmp", "-w", "-DNX=32", "-DNY=32", "-DNZ=32", "-DINTERACTIONS=0",
-           "-DVEXT=0", "-DDEV", "-I" + os.path.join(ROOT, "include"), str(c), "-o", exe,
+           "-DVEXT=0"] + (["-DDEV"] if dev else []) + ["-I" + os.path.join(ROOT, "include"), str(c), "-o", exe,
            "-L" + os.path.dirname(lib), "-lgpe_hip", "-Wl,-rpath," + os.path.dirname(lib),
            "-Wl,-rpath-link,/opt/rocm/lib"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
