@@ -180,6 +180,11 @@ int gpe_engine_set_user_params(gpe_engine_t e, int size, const double *params);
 int gpe_engine_set_quantum_friction_coeff(gpe_engine_t e, double qfcoeff);
 int gpe_engine_set_vortex(gpe_engine_t e, double x0, double y0, int Q);
 int gpe_engine_set_external_potential(gpe_engine_t e, const double *v);
+/* The same from FP64 device memory (nx*ny*nz doubles; no host copy): the
+ * caller's writes to d_v must be complete when the call starts (synchronize
+ * the stream that produced it); the call returns after the engine has read
+ * it.  No counterpart in the reference, whose potentials are host arrays. */
+int gpe_engine_set_external_potential_device(gpe_engine_t e, const double *d_v);
 int gpe_engine_set_psi(gpe_engine_t e, double t, const cuCplx *psi);
 int gpe_engine_get_psi(gpe_engine_t e, double *t, cuCplx *psi);
 int gpe_engine_normalize_psi(gpe_engine_t e);
@@ -283,6 +288,9 @@ int gpe_detect_vortices_noalloc(const void *d_psi, int nx, int ny, int nz, int p
  * friction, launcher variant, time origin) changed after the forward run. */
 int gpe_engine_adjoint_forward(gpe_engine_t e, int nt, const cuCplx *psi_target, int ckpt, double *loss);
 int gpe_engine_adjoint_backward(gpe_engine_t e, double *grad_vext);
+/* The same gradient written to FP64 device memory (N doubles), complete when
+ * the call returns. */
+int gpe_engine_adjoint_backward_device(gpe_engine_t e, double *d_grad_vext);
 
 /* ------------------------------------------------------------------------
  * Ensembles: many independent lattices of possibly different shapes

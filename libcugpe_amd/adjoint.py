@@ -6,8 +6,10 @@ L = 1 - |<psi_t|psi(T)>|^2 / (|psi_t|^2 |psi(0)|^2); its backward runs the
 hand-written discrete adjoint of the split step on the GPU (sqrt(nt)
 checkpoints) and returns dL/dV_ext.  `gaussian_bumps` builds a potential from
 bump amplitudes with torch ops, so autograd carries dL/dV_ext on to the
-amplitudes.  The engine works on host arrays at its C ABI; tensors are moved
-to and from the CPU at this boundary.
+amplitudes.  A potential tensor on the GPU crosses the C ABI as device
+memory (gpe_engine_set_external_potential_device and
+gpe_engine_adjoint_backward_device, after torch's stream has finished
+producing it), a CPU tensor as a host array.
 """
 import numpy as np
 import torch
@@ -18,8 +20,12 @@ from . import engine as E
 class OverlapLoss(torch.autograd.Function):
     @staticmethod
     def forward(ctx, V, eng, psi0, psi_target, nt, ckpt=0):
-        v = V.detach().to("cpu", torch.float64).contiguous().numpy()
-        eng.set_external_potential(v)
+        if V.is_cuda:
+            v = V.detach().to(torch.float64).contiguous()
+            torch.cuda.current_stream(v.device).synchronize()
+            eng.set_external_potential_device(v.data_ptr())
+        else:
+            eng.set_external_potential(V.detach().to(torch.float64).contiguous().numpy())
         eng.set_psi(psi0)
         loss = eng.adjoint_forward(nt, psi_target, ckpt)
         ctx.eng = eng
@@ -32,7 +38,13 @@ class OverlapLoss(torch.autograd.Function):
         if ctx.eng.adjoint_generation != ctx.generation:
             raise RuntimeError("OverlapLoss.backward: the engine ran another adjoint forward after this loss was "
                                "computed; use one engine per live loss")
-        g = torch.from_numpy(ctx.eng.adjoint_backward()).to(device=grad_out.device, dtype=grad_out.dtype)
+        if grad_out.is_cuda:
+            g = torch.empty(ctx.eng.shape, dtype=torch.float64, device=grad_out.device)
+            torch.cuda.current_stream(g.device).synchronize()
+            ctx.eng.adjoint_backward_device(g.data_ptr())
+            g = g.to(grad_out.dtype)
+        else:
+            g = torch.from_numpy(ctx.eng.adjoint_backward()).to(dtype=grad_out.dtype)
         return grad_out * g, None, None, None, None, None
 
 

@@ -1022,6 +1022,33 @@ static int set_varr_slots(gpe_engine_s* e, long slot, const double* v) {
     return 0;
 }
 
+// The VEXT 3 array of every lattice of the engine from FP64 device memory,
+// converted on the engine's stream (no host copy).
+static int set_varr_device(gpe_engine_s* e, const double* d_v) {
+    e->gen++;
+    const size_t es = esize(e);
+    if (!e->varr) HIP_TRY(hipMalloc(&e->varr, e->n * e->batch * es));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (size_t b = 0; b < e->batch; b++) {
+        if (es == 8)
+            HIP_TRY(hipMemcpyAsync((double*)e->varr + b * e->n, d_v, e->n * sizeof(double), hipMemcpyDeviceToDevice,
+                                   e->stream));
+        else
+            launch_f64_to_f32(d_v, (float*)e->varr + b * e->n, e->n, e->stream);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->varr_slots.assign(e->batch, 1);
+    e->varr_set = true;
+    return 0;
+}
+
+int gpe_engine_set_external_potential_device(gpe_engine_t e, const double* d_v) {
+    if (!e) return GPE_ERR_NO_ENGINE;
+    if (!d_v || e->vext != VEXT_ARRAY) return GPE_ERR_ARG;
+    return set_varr_device(e, d_v);
+}
+
 int gpe_engine_set_external_potential(gpe_engine_t e, const double* v) {
     if (!e) return GPE_ERR_NO_ENGINE;
     if (!v) return GPE_ERR_ARG;
@@ -1582,7 +1609,7 @@ int adjoint_forward_t(gpe_engine_s* e, int nt, const cuCplx* tgt, int ckpt, doub
 }
 
 template <typename T>
-int adjoint_backward_t(gpe_engine_s* e, double* grad) {
+int adjoint_backward_t(gpe_engine_s* e, double* grad, bool device_out = false) {
     const size_t n = e->n, cs = sizeof(cplx<T>);
     const int nt = e->adj_nt, C = e->adj_C;
     const int nck = (nt + C - 1) / C;
@@ -1618,7 +1645,8 @@ int adjoint_backward_t(gpe_engine_s* e, double* grad) {
     }
     HIP_TRY(hipMemcpyAsync(e->A, e->adj_fin, n * cs, hipMemcpyDeviceToDevice, e->stream));
     e->it = it_end;
-    HIP_TRY(hipMemcpyAsync(grad, e->adj_grad, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(grad, e->adj_grad, n * sizeof(double),
+                           device_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     return 0;
 }
@@ -1647,6 +1675,13 @@ int gpe_engine_adjoint_backward(gpe_engine_t e, double* grad_vext) {
     // current parameters: they must be those of the forward run
     if (e->adj_gen != e->gen) return GPE_ERR_ARG;
     return DISPATCH(e, adjoint_backward_t, e, grad_vext);
+}
+
+int gpe_engine_adjoint_backward_device(gpe_engine_t e, double* d_grad_vext) {
+    if (!e) return GPE_ERR_NO_ENGINE;
+    if (!d_grad_vext || !e->adj_ready || !adjoint_supported(e)) return GPE_ERR_ARG;
+    if (e->adj_gen != e->gen) return GPE_ERR_ARG;
+    return DISPATCH(e, adjoint_backward_t, e, d_grad_vext, true);
 }
 
 }  // extern "C"

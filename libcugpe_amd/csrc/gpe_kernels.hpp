@@ -22,6 +22,7 @@ int fft_lean_width(int N);
 // element with round-to-nearest (the host buffers of gpe_set_psi / gpe_get_psi)
 void launch_c128_to_c64(const void* src, void* dst, size_t n, hipStream_t s);
 void launch_c64_to_c128(const void* src, void* dst, size_t n, hipStream_t s);
+void launch_f64_to_f32(const double* src, float* dst, size_t n, hipStream_t s);
 
 // one FFT axis pass; returns the number of workgroups launched (0 on error)
 template <typename T>

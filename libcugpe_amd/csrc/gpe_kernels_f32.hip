@@ -20,6 +20,12 @@ __global__ __launch_bounds__(kThreads) void c64_to_c128_kernel(const float2* __r
     }
 }
 
+__global__ __launch_bounds__(kThreads) void f64_to_f32_kernel(const double* __restrict__ src, float* __restrict__ dst,
+                                                             size_t n) {
+    for (size_t i = blockIdx.x * (size_t)kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
+        dst[i] = (float)src[i];
+}
+
 static dim3 convert_grid(size_t n) {
     const size_t b = (n + kThreads - 1) / kThreads;
     return dim3((unsigned)(b < 65536 ? b : 65536));
@@ -29,6 +35,11 @@ void launch_c128_to_c64(const void* src, void* dst, size_t n, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(c128_to_c64_kernel, convert_grid(n), dim3(kThreads), 0, s, (const double2*)src, (float2*)dst,
                        n);
+}
+
+void launch_f64_to_f32(const double* src, float* dst, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(f64_to_f32_kernel, convert_grid(n), dim3(kThreads), 0, s, src, dst, n);
 }
 
 void launch_c64_to_c128(const void* src, void* dst, size_t n, hipStream_t s) {

@@ -57,6 +57,8 @@ def lib():
             "gpe_engine_set_quantum_friction_coeff": [_p, _d],
             "gpe_engine_set_vortex": [_p, _d, _d, _i],
             "gpe_engine_set_external_potential": [_p, _p],
+            "gpe_engine_set_external_potential_device": [_p, _p],
+            "gpe_engine_adjoint_backward_device": [_p, _p],
             "gpe_engine_set_psi": [_p, _d, _p],
             "gpe_engine_get_psi": [_p, _dp, _p],
             "gpe_engine_normalize_psi": [_p],
@@ -189,6 +191,12 @@ class Engine:
     def set_external_potential(self, v):
         a = np.ascontiguousarray(v, dtype=np.float64).reshape(self.shape)
         _check(lib().gpe_engine_set_external_potential(self._h, _ptr(a)), "gpe_set_external_potential")
+
+    def set_external_potential_device(self, ptr):
+        """VEXT 3 array from FP64 device memory at address `ptr` (nx*ny*nz
+        doubles, C order); the producer's writes must be complete."""
+        _check(lib().gpe_engine_set_external_potential_device(self._h, ctypes.c_void_p(int(ptr))),
+               "gpe_engine_set_external_potential_device")
 
     # ------------------------------------------------------ wave function
     def set_psi(self, psi, t=0.0):
@@ -340,6 +348,11 @@ class Engine:
         g = np.empty(self.shape, dtype=np.float64)
         _check(lib().gpe_engine_adjoint_backward(self._h, _ptr(g)), "gpe_engine_adjoint_backward")
         return g
+
+    def adjoint_backward_device(self, ptr):
+        """dL/dV_ext written to FP64 device memory at address `ptr` (N doubles)."""
+        _check(lib().gpe_engine_adjoint_backward_device(self._h, ctypes.c_void_p(int(ptr))),
+               "gpe_engine_adjoint_backward_device")
 
     def autotune(self, steps=3):
         """Time each launcher variant over `steps` real-time steps, keep the fastest; returns it."""

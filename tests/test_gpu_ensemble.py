@@ -103,8 +103,8 @@ def test_ensemble_mean_density_and_autotune():
     ens.autotune(2)
     variants = [g[2] for g in ens.groups()]
     # bits 0-2 of the passes, bit 3 as created, bit 4 the fused whole-step
-    # kernel, bit 5 the y-z plane plan
-    assert all(0 <= v <= 63 for v in variants)
+    # kernel, bit 5 the y-z plane plan, bit 6 the alternative radix schedule
+    assert all(0 <= v <= 127 for v in variants)
     ens.evolve(3)
     for gi, (shape, count, _) in enumerate(ens.groups()):
         members = [i for i in range(len(SHAPES)) if ens.instance(i)[0] == gi]

@@ -225,6 +225,6 @@ def test_fused_autotune_keeps_a_valid_variant():
     ens.autotune(3)
     v = ens.groups()[0][2]
     say("autotuned 24^3 x 64 variant", v)
-    assert 0 <= v <= 31
+    assert 0 <= v <= 127
     ens.evolve(2)
     ens.close()
