@@ -8,10 +8,10 @@ state with 20 z-line vortices of winding +-1.  The state is resident in HBM
 before the timed region starts.
 
 Prints one JSON line with the throughput, the HIP-event roofline of the
-dominant kernel (the x turn-around pass: forward x, K_x, inverse x), the
-energy error of complex64
-against the complex128 engine on the same steps, and the FP64 CPU oracle
-timed on this host as the CPU baseline.
+dominant kernel (the slower of the y and x turn-around passes, two launches
+of each per step), the energy error of complex64 against the complex128
+engine on the same steps, and the FP64 CPU oracle timed on this host as the
+CPU baseline.
 
     python bench.py [--gpus 1] [--steps 1000] [--warmup 50]
 """
@@ -472,24 +472,31 @@ def main():
     sps = a.steps / t
 
     # the same number of steps again with HIP events around every launch of
-    # the dominant kernel, the x turn-around pass (forward x, kinetic factor,
-    # inverse x: two launches per step, the largest share of step time)
-    eng.profile(3, True)
-    t_prof = timed(eng, a.steps)
-    ms_x, n_x = eng.profile_read()
-    eng.profile(3, False)
+    # each turn-around kernel, y (forward y, K_y K_z, inverse y) and x (forward
+    # x, K_x, inverse x): two launches of each per step, together the largest
+    # share of step time; the dominant one of the two is the roofline kernel
+    turn = {}
+    t_prof = None
+    for cls, name in ((3, "x"), (2, "y")):
+        eng.profile(cls, True)
+        tp = timed(eng, a.steps)
+        t_prof = tp if t_prof is None else min(t_prof, tp)
+        ms, cnt = eng.profile_read()
+        eng.profile(cls, False)
+        turn[name] = (ms / max(cnt, 1), cnt)
     # the other pass classes on 1/10 of the steps each
     sub = max(a.steps // 10, 1)
     per_class = {}
-    for cls, name in ((0, "strided_y_and_x"), (1, "z"), (2, "y")):
+    for cls, name in ((0, "strided_y_and_x"), (1, "z")):
         eng.profile(cls, True)
         eng.evolve_qf(sub)
         ms, cnt = eng.profile_read()
         eng.profile(cls, False)
         per_class[name] = ms / max(cnt, 1)
-    per_class["x"] = ms_x / max(n_x, 1)
+    per_class["x"] = turn["x"][0]
+    per_class["y"] = turn["y"][0]
     copy_ms = min(eng.copy_benchmark() for _ in range(5))
-    total_steps = a.warmup + 2 * a.steps + 3 * sub
+    total_steps = a.warmup + 3 * a.steps + 2 * sub
     _, ek, ei, ee = eng.energy()
     e64 = ek + ei + ee
     eng.close()
@@ -498,20 +505,27 @@ def main():
     # the committed counter summary covers 256^3 with the default (split) kinetic plan
     default_plan = a.size == 256 and (VARIANT is None or not (VARIANT & 8))
     bytes_launch = 2 * S  # one read and one write of the state per axis pass
-    avg_ms = ms_x / max(n_x, 1)
-    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    roofline = {
-        "kernel": "fft_pass_kernel<float, %d> x turn-around (forward x, kinetic factor, inverse x)" % a.size,
-        "bound": "hbm",
-        "achieved": achieved,
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS,
-        "traffic": pmc_traffic("x_pass") if default_plan else None,
-        "avg_launch_ms": avg_ms,
-        "launches_timed": n_x,
-        "algorithmic_bytes_per_launch": bytes_launch,
-    }
+
+    def turn_roofline(name):
+        avg_ms, cnt = turn[name]
+        achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+        what = {"x": "x turn-around (forward x, K_x, inverse x)", "y": "y turn-around (forward y, K_y K_z, inverse y)"}
+        return {
+            "kernel": "fft_pass_kernel<float, %d> %s" % (a.size, what[name]),
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": pmc_traffic(name + "_pass") if default_plan else None,
+            "avg_launch_ms": avg_ms,
+            "launches_timed": cnt,
+            "algorithmic_bytes_per_launch": bytes_launch,
+        }
+
+    dominant = max(("x", "y"), key=lambda k: turn[k][0])
+    roofline = turn_roofline(dominant)
+    other_turn = turn_roofline("y" if dominant == "x" else "x")
     strided_ms = per_class["strided_y_and_x"]
     strided = {
         "kernel": "strided y and x passes (4 of 6 launches per step)" if default_plan else "strided y and x passes",
@@ -547,6 +561,7 @@ def main():
         "copy_kernel_GBs": 2 * S / (copy_ms * 1e-3) / 1e9,
         "profiled_region_steps_per_s": a.steps / t_prof,
         "roofline": roofline,
+        "roofline_other_turnaround": other_turn,
     }
 
     if not a.no_c128:

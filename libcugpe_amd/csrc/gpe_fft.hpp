@@ -609,14 +609,20 @@ __device__ __forceinline__ double dens64(double g, double x, double y) { return 
 // MID_EXPT_SEP (AX: the pass axis, -1 when only known at run time)
 enum MidKind { MK_EXPT_64 = 0, MK_EXPT_HL = 1, MK_K2 = 2, MK_VPP = 3, MK_KA = 4, MK_SEP = 5 };
 
+// kx: the x pass's K_x pairs staged in LDS (hi.x, hi.y, lo.x, lo.y), or null
 template <int MK, bool CJ, typename T, int AX = -1>
-__device__ __forceinline__ cplx<T> mid_op_lean_k(const PassArgs<T>& a, const Coord& c, cplx<T> v) {
+__device__ __forceinline__ cplx<T> mid_op_lean_k(const PassArgs<T>& a, const Coord& c, cplx<T> v,
+                                                 const float4* kx = nullptr) {
     if constexpr (MK == MK_SEP) {
         const bool xpass = (AX >= 0) ? (AX == 0) : (a.axis == 0);
         if constexpr (sizeof(T) == 4) {
             // one compensated product per pass with a hi + lo float pair of
             // an FP64 value: K_x in the x pass, the per-pencil K_y K_z in the
             // y pass (two roundings per kinetic step, as in the unsplit form)
+            if (xpass && kx) {
+                const float4 q = kx[c.ix];
+                return cmul_comp(v, mk(q.x, q.y), mk(q.z, q.w));
+            }
             if (xpass) return cmul_comp(v, a.kx_hl[2 * c.ix], a.kx_hl[2 * c.ix + 1]);
             const unsigned yz = c.iy * (unsigned)a.ph.nz + c.iz;
             return cmul_comp(v, a.kyz_hl[2 * yz], a.kyz_hl[2 * yz + 1]);
@@ -1099,7 +1105,8 @@ __device__ __forceinline__ void load_regs(const PassArgs<T>& a, unsigned p0, cpl
 }
 
 template <int MK, bool CJ, typename T, int N, int W, bool CONTIG, int R, int PER, int AX = -1>
-__device__ __forceinline__ void mid_regs_k(const PassArgs<T>& a, unsigned p0, cplx<T> (&v)[PER][R]) {
+__device__ __forceinline__ void mid_regs_k(const PassArgs<T>& a, unsigned p0, cplx<T> (&v)[PER][R],
+                                           const float4* kx = nullptr) {
     constexpr int M = N / R;
     constexpr int NB = W * M;
     // keep the factor-table loads here: hoisted above the first transform they
@@ -1116,7 +1123,7 @@ __device__ __forceinline__ void mid_regs_k(const PassArgs<T>& a, unsigned p0, cp
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const Coord c = elem<AX>(a, pc, (unsigned)(j + r * M));
-                v[q][r] = mid_op_lean_k<MK, CJ, T, AX>(a, c, v[q][r]);
+                v[q][r] = mid_op_lean_k<MK, CJ, T, AX>(a, c, v[q][r], kx);
             }
         }
     }
@@ -1155,7 +1162,8 @@ constexpr int lean_w() {
 // stage's output is in LDS): the remaining stages, the mid op, the optional
 // second transform and the store of tile p0.
 template <typename T, int N, int W, bool CONTIG, int S1, int S2, int MK, bool CJ, int AX, int PV = 0>
-__device__ __forceinline__ void lean_tail(const PassArgs<T>& a, unsigned p0, cplx<T>* lds, const TwE<T>* tw) {
+__device__ __forceinline__ void lean_tail(const PassArgs<T>& a, unsigned p0, cplx<T>* lds, const TwE<T>* tw,
+                                          const float4* kx = nullptr) {
     constexpr FftPlan P = plan_of(N, false, sizeof(T) == 4, PV);
     constexpr int NSG = P.ns;
     constexpr int RL = P.r[NSG - 1];
@@ -1163,7 +1171,7 @@ __device__ __forceinline__ void lean_tail(const PassArgs<T>& a, unsigned p0, cpl
     cplx<T> v[PERL][RL];
     lds_stages<T, N, W, CONTIG, S1, false, 1, NSG - 1, PV>(lds, tw);
     last_stage_regs<T, N, W, CONTIG, S1, RL, PERL>(lds, tw, v);
-    if constexpr (MK >= 0) mid_regs_k<MK, CJ, T, N, W, CONTIG, RL, PERL, AX>(a, p0, v);
+    if constexpr (MK >= 0) mid_regs_k<MK, CJ, T, N, W, CONTIG, RL, PERL, AX>(a, p0, v, kx);
     if constexpr (S2 != 0) {
         constexpr FftPlan Q = plan_of(N, true, sizeof(T) == 4, PV);
         static_assert(Q.r[0] == RL, "reversed plan must start with the last radix");
@@ -1183,7 +1191,8 @@ __device__ __forceinline__ void lean_tail(const PassArgs<T>& a, unsigned p0, cpl
 
 // MK: compile-time kind of the mid op (MidKind), -1 for none; CJ conjugates it
 template <typename T, int N, bool CONTIG, int S1, int S2, int WD, int MK, bool CJ, int AX, int PV = 0>
-__device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, const TwE<T>* tw) {
+__device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, const TwE<T>* tw,
+                                          const float4* kx = nullptr) {
     constexpr int W = lean_w<T, N, WD>();
     constexpr FftPlan P = plan_of(N, false, sizeof(T) == 4, PV);
     constexpr int NSG = P.ns;
@@ -1195,7 +1204,8 @@ __device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, co
         load_regs<T, N, W, CONTIG, false, RL, PERL, AX>(a, p0, v, nullptr);
 #pragma unroll
         for (int q = 0; q < PERL; q++) dft<RL, S1>(v[q]);
-        if constexpr (MK >= 0) mid_regs_k<MK, CJ, T, N, W, CONTIG, RL, PERL, AX>(a, p0, v);
+        if (kx) __syncthreads();  // the LDS factor table is written
+        if constexpr (MK >= 0) mid_regs_k<MK, CJ, T, N, W, CONTIG, RL, PERL, AX>(a, p0, v, kx);
         if constexpr (S2 != 0) {
 #pragma unroll
             for (int q = 0; q < PERL; q++) dft<RL, S2>(v[q]);
@@ -1204,7 +1214,7 @@ __device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, co
     } else {
         first_stage_global<T, N, W, CONTIG, false, S1, P.r[0], AX>(a, p0, lds, nullptr);
         __syncthreads();
-        lean_tail<T, N, W, CONTIG, S1, S2, MK, CJ, AX, PV>(a, p0, lds, tw);
+        lean_tail<T, N, W, CONTIG, S1, S2, MK, CJ, AX, PV>(a, p0, lds, tw, kx);
     }
 }
 
@@ -1514,7 +1524,16 @@ void fft_pass_kernel(const PassArgs<T> a) {
     if constexpr (!FULL) {
         constexpr int W = lean_w<T, N, WD>();
         __shared__ cplx<T> lds[lds_elems<N, W, CONTIG>()];
-        lean_body<T, N, CONTIG, S1, S2, WD, MK, CJ, (CONTIG ? 2 : AX), PV>(a, lds, tw);
+        // the x turn-around's K_x pairs in LDS: read from global memory per
+        // point, their 16-byte loads cost the pass ~15 us of its ~66 at 256^3
+        // complex64 (measured)
+        constexpr bool KXL = sizeof(T) == 4 && MK == MK_SEP && AX == 0 && !CONTIG;
+        __shared__ float4 kxs[KXL ? N : 1];
+        if constexpr (KXL) {
+            const float4* src = reinterpret_cast<const float4*>(a.kx_hl);
+            for (int i = threadIdx.x; i < N; i += kThreads) kxs[i] = src[i];
+        }
+        lean_body<T, N, CONTIG, S1, S2, WD, MK, CJ, (CONTIG ? 2 : AX), PV>(a, lds, tw, KXL ? kxs : nullptr);
     } else {
         __shared__ FullLds<T, N, CONTIG> L;
         __shared__ double red[kThreads / 64];
