@@ -22,7 +22,9 @@
 // transforms, both free of bank conflicts.  The physics runs in registers on
 // the outputs of the inverse y transform's last stage, in the chunked form of
 // the z-wave passes (per element with the user hooks), and the kinetic factor
-// K_y K_z / N and the store on those of the forward y transform's last stage.
+// K_y K_z / N and the store on those of the forward y transform's last stage,
+// the factors K_y and K_z / N taken from LDS tables (complex64: two
+// compensated products with float hi + lo pairs).
 #pragma once
 
 #include "gpe_zwave.hpp"
@@ -43,7 +45,23 @@ struct PlaneLds {
     static constexpr int P = L + 1;
     cplx<T> x[L * P];
     TwE<T> tw[L];
+    TwE<T> ky[L];  // K_y and K_z / N (complex64: float hi + lo pairs of the FP64 factors)
+    TwE<T> kz[L];
 };
+
+// element (iy, iz) times K_y[iy] K_z[iz] / N from the LDS tables: complex64
+// as two compensated products, complex128 as the FP64 products of the y
+// turn-around pass
+template <typename T, int L>
+__device__ __forceinline__ cplx<T> plane_kin(const PlaneLds<T, L>& S, unsigned iy, unsigned iz, cplx<T> v) {
+    if constexpr (sizeof(T) == 4) {
+        const float4 y = S.ky[iy].t, z = S.kz[iz].t;
+        v = cmul_comp(v, mk(y.x, y.y), mk(y.z, y.w));
+        return cmul_comp(v, mk(z.x, z.y), mk(z.z, z.w));
+    } else {
+        return cmul(cmul(v, S.ky[iy].t), S.kz[iz].t);
+    }
+}
 
 // radix-R Stockham stage (span NS) of L pencils of L points; element i of
 // pencil w sits at w * WS + i * IS
@@ -193,8 +211,15 @@ void plane_kernel(const PassArgs<T> a) {
         if constexpr (sizeof(T) == 4) {
             const cplx<float> h = a.tw[i], lo = a.twlo[i];
             S.tw[i].t = make_float4(h.x, h.y, lo.x, lo.y);
+            // hi + lo float pairs of the FP64 kinetic factors
+            const cplx<double> ky = a.Ky[i], kz = a.Kz[i];
+            const float yh = (float)ky.x, yi = (float)ky.y, zh = (float)kz.x, zi = (float)kz.y;
+            S.ky[i].t = make_float4(yh, yi, (float)(ky.x - (double)yh), (float)(ky.y - (double)yi));
+            S.kz[i].t = make_float4(zh, zi, (float)(kz.x - (double)zh), (float)(kz.y - (double)zi));
         } else {
             S.tw[i].t = a.tw[i];
+            S.ky[i].t = a.Ky[i];
+            S.kz[i].t = a.Kz[i];
         }
     }
     const unsigned nx = (unsigned)a.ph.nx;
@@ -339,7 +364,9 @@ void plane_kernel(const PassArgs<T> a) {
     plane_stages<T, L, P, 1, -1, 0, NS>(S.x, S.tw);      // forward z
     plane_stages<T, L, 1, P, -1, 0, NS - 1>(S.x, S.tw);  // forward y but its last stage
     plane_last_y<T, L, -1>(S.x, S.tw, v);
-    // K_y K_z / N of the split kinetic factor, as the y turn-around pass applies it, and the store
+    // K_y K_z / N of the split kinetic factor from the LDS tables (per-point
+    // loads of a K_y K_z table from global memory had cost a plane pass
+    // 11-14%, measured), and the store
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         const int b = threadIdx.x + q * kPlaneThreads;
@@ -347,12 +374,8 @@ void plane_kernel(const PassArgs<T> a) {
             const int w = b % L, j = b / L;
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                Coord c;
-                c.ix = ix;
-                c.iy = (unsigned)(j + r * M);
-                c.iz = (unsigned)w;
-                c.off = base + c.iy * (unsigned)L + c.iz;
-                a.dst[c.off] = mid_op_lean_k<MK_SEP, false, T, 1>(a, c, v[q][r]);
+                const unsigned iy = (unsigned)(j + r * M);
+                a.dst[base + iy * (unsigned)L + (unsigned)w] = plane_kin<T, L>(S, iy, (unsigned)w, v[q][r]);
             }
         }
     }
