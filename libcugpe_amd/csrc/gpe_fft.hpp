@@ -86,16 +86,24 @@ constexpr int tile_w() {
     return p2;
 }
 
+// Row pitch of a strided tile: rows of 16 or more elements are stored
+// unpadded (a row already spans the lanes that share an LDS cycle, and at
+// 256^3 complex64 the saved 2 KB lets four x turn-around workgroups share a
+// CU); narrower rows keep one pad element against first-stage write conflicts.
+template <int W>
+constexpr int row_pitch() {
+    return W >= 16 ? W : W + 1;
+}
 template <int N, int W, bool CONTIG>
 __device__ __forceinline__ int lidx(int w, int i) {
     if constexpr (CONTIG)
         return w * (N + N / 16) + i + (i >> 4);
     else
-        return i * (W + 1) + w;
+        return i * row_pitch<W>() + w;
 }
 template <int N, int W, bool CONTIG>
 constexpr int lds_elems() {
-    return CONTIG ? W * (N + N / 16) : N * (W + 1);
+    return CONTIG ? W * (N + N / 16) : N * row_pitch<W>();
 }
 
 // ---------------------------------------------------- radix butterflies
