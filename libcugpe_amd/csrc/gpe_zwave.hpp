@@ -239,7 +239,10 @@ __device__ __forceinline__ void mul_exp_chunk(cplx<double> (&v)[CH], const doubl
 #endif
 // resident waves per SIMD the compiler must allow (VGPR budget 512 / waves):
 // complex64 4, or ZW_WAVES_PHYS for the passes with physics between transforms
-// and for the imaginary-time corrector
+// and for the imaginary-time corrector, except the corrector-end pass
+// (MID_VSTEP3), which fits 128 VGPRs without spills at every length and runs
+// at 4 (256^3: 2780 -> 2818 steps/s; MID_VSTEP2 spills at 4, and with chunks
+// of 2 it fits but loses 11 us)
 // (512 points: 32 registers of data per lane and three workgroups per CU by
 // LDS; 2 waves with physics, 3 without)
 template <typename T, int N, int SHAPE, int LM, int MM>
@@ -247,7 +250,7 @@ constexpr int zw_waves() {
     constexpr bool phys = (SHAPE == SHAPE_IF && MM != MID_NONE) || LM == LOAD_PART2;
     if constexpr (sizeof(T) != 4) return 2;
     else if constexpr (N > 256) return (phys || MM != MID_NONE || LM != LOAD_PLAIN) ? 2 : 3;
-    else return phys ? ZW_WAVES_PHYS : 4;
+    else return phys ? (MM == MID_VSTEP3 ? 4 : ZW_WAVES_PHYS) : 4;
 }
 
 template <typename T, int N, int SHAPE, int LM, int MM>
