@@ -276,6 +276,14 @@ def adjoint_bench(E, T, oracle_steps, fd_bumps=(0, 101, 307)):
     L.backward()
     t2 = perf_counter()
     grad = amp.grad.numpy().copy()
+    # the same gradient from a complex128 engine through the same wrapper
+    eng128 = E.Engine(shape, 1.0, 0.0, dt_a, float(n), E.UFG, E.VARRAY, 0, E.C128)
+    amp = torch.tensor(amp0, requires_grad=True)
+    t4 = perf_counter()
+    A.overlap_loss(A.gaussian_bumps(amp, centres, 1.5, shape), eng128, psi0, psit, T).backward()
+    t5 = perf_counter()
+    grad128 = amp.grad.numpy().copy()
+    eng128.close()
     scale = float(np.max(np.abs(grad)))
     h = 1e-4
     fd = []
@@ -284,7 +292,9 @@ def adjoint_bench(E, T, oracle_steps, fd_bumps=(0, 101, 307)):
         ap[j] += h
         am[j] -= h
         d = (loss128(ap) - loss128(am)) / (2 * h)
-        fd.append({"bump": j, "fd_c128": d, "adjoint_c64": float(grad[j]), "abs_err_over_max_grad": abs(d - grad[j]) / scale})
+        fd.append({"bump": j, "fd_c128": d, "adjoint_c64": float(grad[j]), "adjoint_c128": float(grad128[j]),
+                   "abs_err_over_max_grad": abs(d - grad[j]) / scale,
+                   "abs_err_c128_over_max_grad": abs(d - grad128[j]) / scale})
     g.close()
     # the oracle's FP64 adjoint on the first oracle_steps steps against the engine's
     from oracle import pyoracle
@@ -310,7 +320,12 @@ def adjoint_bench(E, T, oracle_steps, fd_bumps=(0, 101, 307)):
         "forward_s": t1 - t0,
         "backward_s": t2 - t1,
         "fd_check": fd,
+        "max_grad": scale,
         "max_abs_err_vs_fd_over_max_grad": max(x["abs_err_over_max_grad"] for x in fd),
+        "max_abs_err_vs_fd": max(x["abs_err_over_max_grad"] for x in fd) * scale,
+        "c128": {"gradient_wall_s": t5 - t4,
+                 "max_abs_err_vs_fd_over_max_grad": max(x["abs_err_c128_over_max_grad"] for x in fd),
+                 "max_abs_err_vs_fd": max(x["abs_err_c128_over_max_grad"] for x in fd) * scale},
         "oracle_sample": {
             "steps": oracle_steps,
             "loss_abs_err": abs(Lg - Lo),
