@@ -628,10 +628,7 @@ __device__ __forceinline__ cplx<T> mid_op_lean_k(const PassArgs<T>& a, const Coo
             // an FP64 value: K_x in the x pass, the per-pencil K_y K_z in the
             // y pass (two roundings per kinetic step, as in the unsplit form)
             if (xpass && kx) {
-                // read through an LDS-qualified pointer: the generic one
-                // (compared with null above) compiles to flat loads
-                using lds_f4 = __attribute__((address_space(3))) const float4;
-                const float4 q = ((lds_f4*)kx)[c.ix];
+                const float4 q = kx[c.ix];
                 return cmul_comp(v, mk(q.x, q.y), mk(q.z, q.w));
             }
             if (xpass) return cmul_comp(v, a.kx_hl[2 * c.ix], a.kx_hl[2 * c.ix + 1]);
