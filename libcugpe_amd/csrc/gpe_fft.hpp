@@ -617,7 +617,8 @@ __device__ __forceinline__ double dens64(double g, double x, double y) { return 
 // MID_EXPT_SEP (AX: the pass axis, -1 when only known at run time)
 enum MidKind { MK_EXPT_64 = 0, MK_EXPT_HL = 1, MK_K2 = 2, MK_VPP = 3, MK_KA = 4, MK_SEP = 5 };
 
-// kx: the x pass's K_x pairs staged in LDS (hi.x, hi.y, lo.x, lo.y), or null
+// kx: the pass's factor staged in LDS, or null: complex64 the x pass's K_x
+// pairs (hi.x, hi.y, lo.x, lo.y), complex128 K_x (x pass) or K_y (y pass)
 template <int MK, bool CJ, typename T, int AX = -1>
 __device__ __forceinline__ cplx<T> mid_op_lean_k(const PassArgs<T>& a, const Coord& c, cplx<T> v,
                                                  const float4* kx = nullptr) {
@@ -636,7 +637,11 @@ __device__ __forceinline__ cplx<T> mid_op_lean_k(const PassArgs<T>& a, const Coo
             return cmul_comp(v, a.kyz_hl[2 * yz], a.kyz_hl[2 * yz + 1]);
         } else {
             cplx<double> w = mk((double)v.x, (double)v.y);
-            if (xpass) {
+            if (kx) {
+                // complex128: K_x (x pass) or K_y (y pass) staged in LDS
+                w = cmul(w, reinterpret_cast<const cplx<double>*>(kx)[xpass ? c.ix : c.iy]);
+                if (!xpass) w = cmul(w, a.Kz[c.iz]);
+            } else if (xpass) {
                 w = cmul(w, a.Kx[c.ix]);
             } else {
                 w = cmul(w, a.Ky[c.iy]);
@@ -1534,11 +1539,17 @@ void fft_pass_kernel(const PassArgs<T> a) {
         __shared__ cplx<T> lds[lds_elems<N, W, CONTIG>()];
         // the x turn-around's K_x pairs in LDS: read from global memory per
         // point, their 16-byte loads cost the pass ~15 us of its ~66 at 256^3
-        // complex64 (measured)
-        constexpr bool KXL = sizeof(T) == 4 && MK == MK_SEP && AX == 0 && !CONTIG;
+        // complex64 (measured).  complex128 stages K_x or K_y (16 bytes per
+        // entry) up to 256 points, where the extra LDS costs no workgroup per
+        // CU: 256^3 y pass 110.5 -> 105.0 us, x 97.7 -> 95.1 us.  complex64's
+        // y pass keeps its per-(iy, iz) table (K_y in LDS with a second
+        // product measured slower)
+        constexpr bool KXL = MK == MK_SEP && !CONTIG &&
+                             (sizeof(T) == 4 ? AX == 0 : ((AX == 0 || AX == 1) && N <= 256));
         __shared__ float4 kxs[KXL ? N : 1];
         if constexpr (KXL) {
-            const float4* src = reinterpret_cast<const float4*>(a.kx_hl);
+            const float4* src = reinterpret_cast<const float4*>(
+                sizeof(T) == 4 ? (const void*)a.kx_hl : (AX == 0 ? (const void*)a.Kx : (const void*)a.Ky));
             for (int i = threadIdx.x; i < N; i += kThreads) kxs[i] = src[i];
         }
         lean_body<T, N, CONTIG, S1, S2, WD, MK, CJ, (CONTIG ? 2 : AX), PV>(a, lds, tw, KXL ? kxs : nullptr);
