@@ -76,7 +76,9 @@ template <typename T, int N>
 constexpr int tile_w() {
     // complex64 512: 16 columns (128-byte rows) rather than 8; complex128 128
     // and 256: 32 and 16 columns, so that every thread owns a butterfly of
-    // the radix-16 first stage (256-byte rows)
+    // the radix-16 first stage (256-byte rows); complex128 512 keeps 4
+    // columns (8 measured slower: 62 against 70 steps/s at 512^3, two
+    // workgroups per CU instead of three)
     int base = (sizeof(T) == 4) ? (N >= 512 ? 8192 : 4096) : ((N == 128 || N == 256) ? 4096 : 2048);
     int w = base / N;
     if (w < 1) w = 1;
@@ -86,24 +88,24 @@ constexpr int tile_w() {
     return p2;
 }
 
-// Row pitch of a strided tile: rows of 16 or more elements are stored
+// Row pitch of a strided tile: rows of 128 bytes or more are stored
 // unpadded (a row already spans the lanes that share an LDS cycle, and at
 // 256^3 complex64 the saved 2 KB lets four x turn-around workgroups share a
-// CU); narrower rows keep one pad element against first-stage write conflicts.
-template <int W>
+// CU); shorter rows keep one pad element against first-stage write conflicts.
+template <typename T, int W>
 constexpr int row_pitch() {
-    return W >= 16 ? W : W + 1;
+    return W * (int)sizeof(cplx<T>) >= 128 ? W : W + 1;
 }
-template <int N, int W, bool CONTIG>
+template <typename T, int N, int W, bool CONTIG>
 __device__ __forceinline__ int lidx(int w, int i) {
     if constexpr (CONTIG)
         return w * (N + N / 16) + i + (i >> 4);
     else
-        return i * row_pitch<W>() + w;
+        return i * row_pitch<T, W>() + w;
 }
-template <int N, int W, bool CONTIG>
+template <typename T, int N, int W, bool CONTIG>
 constexpr int lds_elems() {
-    return CONTIG ? W * (N + N / 16) : N * row_pitch<W>();
+    return CONTIG ? W * (N + N / 16) : N * row_pitch<T, W>();
 }
 
 // ---------------------------------------------------- radix butterflies
@@ -383,7 +385,7 @@ __device__ __forceinline__ void fft_stage(cplx<T>* lds, const TwE<T>* tw) {
             if constexpr (CONTIG) { w = b / M; j = b % M; }
             else { j = b / W; w = b % W; }
 #pragma unroll
-            for (int r = 0; r < R; r++) v[q][r] = lds[lidx<N, W, CONTIG>(w, j + r * M)];
+            for (int r = 0; r < R; r++) v[q][r] = lds[lidx<T, N, W, CONTIG>(w, j + r * M)];
         }
     }
     __syncthreads();
@@ -402,7 +404,7 @@ __device__ __forceinline__ void fft_stage(cplx<T>* lds, const TwE<T>* tw) {
             dft<R, S>(v[q]);
             const int idxD = (j / NS) * NS * R + k;
 #pragma unroll
-            for (int r = 0; r < R; r++) lds[lidx<N, W, CONTIG>(w, idxD + r * NS)] = v[q][r];
+            for (int r = 0; r < R; r++) lds[lidx<T, N, W, CONTIG>(w, idxD + r * NS)] = v[q][r];
         }
     }
     __syncthreads();
@@ -1038,7 +1040,7 @@ __device__ __forceinline__ void first_stage_global(const PassArgs<T>& a, unsigne
             }
             dft<R, S>(v);
 #pragma unroll
-            for (int r = 0; r < R; r++) lds[lidx<N, W, CONTIG>(w, j * R + r)] = v[r];
+            for (int r = 0; r < R; r++) lds[lidx<T, N, W, CONTIG>(w, j * R + r)] = v[r];
         }
     }
 }
@@ -1056,7 +1058,7 @@ __device__ __forceinline__ void first_stage_regs(cplx<T> (&v)[PER][R], cplx<T>* 
             bmap<M, W, CONTIG>(b, w, j);
             dft<R, S>(v[q]);
 #pragma unroll
-            for (int r = 0; r < R; r++) lds[lidx<N, W, CONTIG>(w, j * R + r)] = v[q][r];
+            for (int r = 0; r < R; r++) lds[lidx<T, N, W, CONTIG>(w, j * R + r)] = v[q][r];
         }
     }
 }
@@ -1084,7 +1086,7 @@ __device__ __forceinline__ void last_stage_regs(const cplx<T>* lds, const TwE<T>
             int w, j;
             bmap<M, W, CONTIG>(b, w, j);
 #pragma unroll
-            for (int r = 0; r < R; r++) v[q][r] = lds[lidx<N, W, CONTIG>(w, j + r * M)];
+            for (int r = 0; r < R; r++) v[q][r] = lds[lidx<T, N, W, CONTIG>(w, j + r * M)];
 #pragma unroll
             for (int r = 1; r < R; r++) v[q][r] = tw_mul<S>(v[q][r], tw[j * r]);  // twiddle w_N^(j r)
             dft<R, S>(v[q]);
@@ -1289,7 +1291,7 @@ __device__ __forceinline__ void first_stage_stage(const cplx<float>* stage, cplx
             for (int r = 0; r < R; r++) v[r] = stage[(j + r * M) * W + w];
             dft<R, S>(v);
 #pragma unroll
-            for (int r = 0; r < R; r++) lds[lidx<N, W, false>(w, j * R + r)] = v[r];
+            for (int r = 0; r < R; r++) lds[lidx<float, N, W, false>(w, j * R + r)] = v[r];
         }
     }
 }
@@ -1303,7 +1305,7 @@ void fft_pipe_kernel(const PassArgs<float> a) {
     constexpr int S2 = (SHAPE == SHAPE_FI) ? 1 : (SHAPE == SHAPE_IF) ? -1 : 0;
     __shared__ TwE<float> tw[N];
     __shared__ cplx<float> stage[W * N];
-    __shared__ cplx<float> lds[lds_elems<N, W, false>()];
+    __shared__ cplx<float> lds[lds_elems<float, N, W, false>()];
     for (int i = threadIdx.x; i < N; i += kThreads) {
         const cplx<float> h = a.tw[i], lo = a.twlo[i];
         tw[i].t = make_float4(h.x, h.y, lo.x, lo.y);
@@ -1344,7 +1346,7 @@ constexpr int full_tile_w() {
 template <typename T, int N, bool CONTIG>
 struct FullLds {
     static constexpr int W = full_tile_w<T, N, CONTIG>();
-    static constexpr int TILE = lds_elems<N, W, CONTIG>();
+    static constexpr int TILE = lds_elems<T, N, W, CONTIG>();
     static constexpr int SIDE = CONTIG ? TILE : 1;   // side tiles only in z passes
     cplx<T> main[TILE];
     cplx<T> aux[SIDE];
@@ -1385,7 +1387,7 @@ __device__ __forceinline__ ElemIn<T> elem_in(const PassArgs<T>& a, const FullLds
         if constexpr (UH) {
             if (a.ph.vext == VEXT_USER) e.vx1 = vext<UH>(a.ph, e.c.ix, e.c.iy, e.c.iz, e.c.off, it + 1);
         }
-        const int li = lidx<N, FullLds<T, N, CONTIG>::W, CONTIG>(w, i);
+        const int li = lidx<T, N, FullLds<T, N, CONTIG>::W, CONTIG>(w, i);
         e.aux = L.aux[li];
         e.raux = L.raux[li];
     } else {
@@ -1446,7 +1448,7 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
             if constexpr (CONTIG) { w = e / N; i = e % N; }
             else { i = e / W; w = e % W; }
             const unsigned p = p0 + (unsigned)w;
-            const int li = lidx<N, W, CONTIG>(w, i);
+            const int li = lidx<T, N, W, CONTIG>(w, i);
             if (p < a.M) {
                 const Pencil pc = make_pencil(a, N, p);
                 const unsigned off = pc.base + (unsigned)i * pc.stride;
@@ -1471,7 +1473,7 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
                 if constexpr (CONTIG) { w = e / N; i = e % N; }
                 else { i = e / W; w = e % W; }
                 if (L.ok[w]) {
-                    const int li = lidx<N, W, CONTIG>(w, i);
+                    const int li = lidx<T, N, W, CONTIG>(w, i);
                     L.main[li] = load_op2<UH>(a, elem_in<UH>(a, L, w, i, sep, vscale, it), L.main[li], acc);
                 }
             }
@@ -1494,7 +1496,7 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
             if constexpr (CONTIG) { w = e / N; i = e % N; }
             else { i = e / W; w = e % W; }
             if (L.ok[w]) {
-                const int li = lidx<N, W, CONTIG>(w, i);
+                const int li = lidx<T, N, W, CONTIG>(w, i);
                 L.main[li] = mid_op2<UH>(a, elem_in<UH>(a, L, w, i, sep, vscale, it), L.main[li], acc);
             }
         }
@@ -1511,7 +1513,7 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
                 int w, i;
                 if constexpr (CONTIG) { w = e / N; i = e % N; }
                 else { i = e / W; w = e % W; }
-                if (L.ok[w]) a.dst[L.base[w] + (unsigned)i * axis_stride(a)] = L.main[lidx<N, W, CONTIG>(w, i)];
+                if (L.ok[w]) a.dst[L.base[w] + (unsigned)i * axis_stride(a)] = L.main[lidx<T, N, W, CONTIG>(w, i)];
             }
         }
     }
@@ -1536,7 +1538,7 @@ void fft_pass_kernel(const PassArgs<T> a) {
     constexpr int S2 = (SHAPE == SHAPE_FI) ? 1 : (SHAPE == SHAPE_IF) ? -1 : 0;
     if constexpr (!FULL) {
         constexpr int W = lean_w<T, N, WD>();
-        __shared__ cplx<T> lds[lds_elems<N, W, CONTIG>()];
+        __shared__ cplx<T> lds[lds_elems<T, N, W, CONTIG>()];
         // the x turn-around's K_x pairs in LDS: read from global memory per
         // point, their 16-byte loads cost the pass ~15 us of its ~66 at 256^3
         // complex64 (measured).  complex128 stages K_x or K_y (16 bytes per
