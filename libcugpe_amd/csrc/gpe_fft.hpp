@@ -1204,6 +1204,25 @@ __device__ __forceinline__ void lean_tail(const PassArgs<T>& a, unsigned p0, cpl
     }
 }
 
+// Tile of workgroup b.  Strided tiles whose rows are shorter than a 128-byte
+// line share each line with the next G - 1 tiles.  Workgroups are dispatched
+// round-robin over the 8 XCDs, each with its own L2, so those G tiles go to
+// workgroups b, b + 8, ..., b + 8 (G - 1) of one XCD rather than to
+// neighbouring workgroups on different XCDs (a trailing partial group keeps
+// the plain order).
+template <typename T, int W, bool CONTIG>
+__device__ __forceinline__ unsigned tile_of_block(unsigned b) {
+    constexpr int RB = W * (int)sizeof(cplx<T>);
+    if constexpr (CONTIG || RB >= 128) {
+        return b;
+    } else {
+        constexpr unsigned G = 128 / RB, X = 8, GX = G * X;
+        const unsigned q = b / GX, r = b - q * GX;
+        if ((q + 1) * GX > gridDim.x) return b;
+        return q * GX + (r % X) * G + r / X;
+    }
+}
+
 // MK: compile-time kind of the mid op (MidKind), -1 for none; CJ conjugates it
 template <typename T, int N, bool CONTIG, int S1, int S2, int WD, int MK, bool CJ, int AX, int PV = 0>
 __device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, const TwE<T>* tw,
@@ -1213,7 +1232,7 @@ __device__ __forceinline__ void lean_body(const PassArgs<T>& a, cplx<T>* lds, co
     constexpr int NSG = P.ns;
     constexpr int RL = P.r[NSG - 1];
     constexpr int PERL = (W * (N / RL) + kThreads - 1) / kThreads;
-    const unsigned p0 = blockIdx.x * (unsigned)W;
+    const unsigned p0 = tile_of_block<T, W, CONTIG>(blockIdx.x) * (unsigned)W;
     if constexpr (NSG == 1) {
         cplx<T> v[PERL][RL];
         load_regs<T, N, W, CONTIG, false, RL, PERL, AX>(a, p0, v, nullptr);
