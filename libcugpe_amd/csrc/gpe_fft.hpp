@@ -1329,6 +1329,14 @@ void fft_pipe_kernel(const PassArgs<float> a) {
         const cplx<float> h = a.tw[i], lo = a.twlo[i];
         tw[i].t = make_float4(h.x, h.y, lo.x, lo.y);
     }
+    // the x pass's K_x pairs in LDS, as in the lean passes (ordered before
+    // their first use by the loop's first barrier)
+    constexpr bool KXL = MK == MK_SEP && AX == 0;
+    __shared__ float4 kxs[KXL ? N : 1];
+    if constexpr (KXL) {
+        const float4* src = reinterpret_cast<const float4*>(a.kx_hl);
+        for (int i = threadIdx.x; i < N; i += kThreads) kxs[i] = src[i];
+    }
     const unsigned ntiles = a.M / (unsigned)W;
     unsigned t = blockIdx.x;
     if (t < ntiles) pipe_fetch<N, W, AX>(a, t * (unsigned)W, stage);
@@ -1340,7 +1348,7 @@ void fft_pipe_kernel(const PassArgs<float> a) {
         first_stage_stage<N, W, S1, P.r[0]>(stage, lds);
         __syncthreads();  // the staging buffer is free, the tile is in LDS
         if (t + gridDim.x < ntiles) pipe_fetch<N, W, AX>(a, (t + gridDim.x) * (unsigned)W, stage);
-        lean_tail<float, N, W, false, S1, S2, MK, CJ, AX>(a, t * (unsigned)W, lds, tw);
+        lean_tail<float, N, W, false, S1, S2, MK, CJ, AX>(a, t * (unsigned)W, lds, tw, KXL ? kxs : nullptr);
     }
 }
 
