@@ -326,6 +326,40 @@ def test_launcher_variants_match_oracle(oracle, variant, shape):
         g.close()
 
 
+@pytest.mark.parametrize("prec,variant,shape", [
+    (E.C64, 2, (256, 64, 8)),    # x rows of 8 complex64 (64 B): pairs, 4 full groups
+    (E.C64, 2, (256, 24, 8)),    # 24 tiles: one full group and 8 in plain order
+    (E.C128, 0, (512, 16, 8)),   # x rows of 4 complex128 (64 B): pairs
+    (E.C128, 2, (512, 16, 8)),   # x rows of 2 complex128 (32 B): groups of 4
+    (E.C128, 0, (512, 12, 8)),   # 24 tiles: one full group, 8 in plain order
+    (E.C128, 2, (16, 512, 12)),  # y axis, 96 tiles: 3 full groups of 32
+])
+def test_short_row_tile_order_matches_oracle(oracle, prec, variant, shape):
+    """Strided tiles with rows shorter than 128 bytes are handed to workgroups
+    in line-sharing groups per XCD (tile_of_block): every tile is still
+    transformed exactly once, so the step follows the oracle (complex128 to
+    the parity bound, complex64 to its rounding), with full and trailing
+    partial groups."""
+    say("short-row tiles", prec, variant, shape)
+    g = E.Engine(shape, 1.0, 0.0, 0.02, 600.0, E.CONTACT, E.COSINE, 0, prec)
+    o = oracle.OracleEngine(shape, 1.0, 0.0, 0.02, 600.0, E.CONTACT, E.COSINE, 0)
+    for x in (g, o):
+        x.set_user_params([0.1, 2.0, 0.0, 0.02])
+    g.set_variant(variant)
+    psi = seeded_state(shape, 29)
+    for x in (g, o):
+        x.set_psi(psi)
+        x.normalize_psi()
+        x.evolve_qf(4)
+    err = rel_err(g.get_psi()[0], o.get_psi()[0])
+    assert err < (TOL128 if prec == E.C128 else 2e-5), err
+    g.fft(-1)
+    o_k = np.fft.fftn(o.get_psi()[0])
+    assert rel_err(g.get_psi()[0], o_k) < (1e-12 if prec == E.C128 else 2e-5)
+    g.close()
+    o.close()
+
+
 @pytest.mark.parametrize("mode", ["rte", "ite"])
 @pytest.mark.parametrize("shape", [(32, 24, 16), (24, 16, 12)])
 def test_dipolar_matches_oracle(oracle, mode, shape):
