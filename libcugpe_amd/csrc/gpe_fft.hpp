@@ -1374,6 +1374,10 @@ constexpr int full_tile_w() {
     // complex128 128 and 256: the 2048-element tiles (the wider lean tiles
     // measured slower here: 137 -> 197 us for the energy's kinetic pass)
     if (sizeof(T) == 8 && (N == 128 || N == 256) && w > 1) w /= 2;
+    // complex128 512 along z (no wave split there): one pencil per workgroup,
+    // 34 KB of LDS, so that four workgroups share a CU at the four-wave bound
+    // of fft_pass_kernel (512^3: 70.5 -> 87.6 steps/s)
+    if (sizeof(T) == 8 && N == 512 && CONTIG) w = 1;
     return w;
 }
 
@@ -1553,9 +1557,13 @@ __device__ __forceinline__ void full_body(const PassArgs<T>& a, FullLds<T, N, CO
     }
 }
 
+// complex64 at 4 waves per SIMD, complex128 at 2, except the complex128
+// 512-point z physics passes of the built-in physics: bounded to 4 waves they
+// need 75-77 VGPRs and no spills (at 2 the compiler spends ~190); with user
+// hooks they would spill, and at 96 and 192 points (wider tiles) too
 template <typename T, int N, bool CONTIG, bool FULL, int SHAPE, int WD = 1, int MK = -1, bool CJ = false, int AX = -1,
           bool UH = false, int PV = 0>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : ((FULL && CONTIG && N == 512 && !UH) ? 4 : 2))))
 void fft_pass_kernel(const PassArgs<T> a) {
     __shared__ TwE<T> tw[N];
     for (int i = threadIdx.x; i < N; i += kThreads) {
